@@ -1,0 +1,397 @@
+#!/usr/bin/env python3
+"""Throughput of the sparse hot path on one MI355X.
+
+Default workload: power-law FP64 SpMV, y(i) = A(i,j) * x(j), A CSR
+2,000,000 x 2,000,000 with Zipf(2.0) row lengths on [1, 1e5] rescaled to mean
+16 (~32M entries), uniform distinct columns, values and x U[-1,1] (synthetic,
+seed-generated).  A "step" is one compute() over that matrix with every input
+already resident in HBM.
+
+Prints one JSON line (metric, value, roofline, cpu_baseline, ...).  The CPU
+baseline is taco's own generated OpenMP code (the real taco, compiled from the
+reference sources into oracle/_ref) on the host cores, or the oracle's C
+restatement of it when that library is absent; its output also checks the GPU
+result.
+
+Other workloads (--workload): spmm_f64, spmm_f32, spmm_bf16 (same A, dense B
+2,000,000 x 64), spgemm (500,000^2, 8 entries per row), spmv_uniform (100,000^2,
+10 entries per row).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def trimmed_mean(ts):
+    ts = sorted(ts)
+    cut = int(len(ts) * 0.1)
+    core = ts[cut:len(ts) - cut] if len(ts) - 2 * cut > 0 else ts
+    return float(np.mean(core))
+
+
+# ---------------------------------------------------------------- CPU leg --
+
+def cpu_baseline_spmv(A, x, reps):
+    """Times taco's CPU SpMV on the host (before the GPU is touched)."""
+    import oracle
+    cores = host_cores()
+    if oracle.ref_available():
+        y, ts = oracle.ref_spmv(A, x, threads=cores, reps=reps)
+        kind = "reference"
+    else:
+        oracle.set_threads(cores)
+        oracle.spmv(A, x)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            y = oracle.spmv(A, x)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        kind = "port"
+    return y, list(ts), cores, kind
+
+
+def cpu_baseline_spmm(A, B, reps):
+    import oracle
+    cores = host_cores()
+    if oracle.ref_available():
+        C, ts = oracle.ref_spmm(A, B, threads=cores, reps=reps)
+        kind = "reference"
+    else:
+        oracle.set_threads(cores)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            C = oracle.spmm(A, B)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        kind = "port"
+    return C, list(ts), cores, kind
+
+
+def cpu_baseline_spgemm(A, B, reps):
+    import oracle
+    cores = host_cores()
+    if oracle.ref_available():
+        pos, crd, vals, ta, tc = oracle.ref_spgemm(A, B, threads=cores, reps=reps)
+        kind = "reference"
+        ts = [a + c for a, c in zip(ta, tc)]
+    else:
+        oracle.set_threads(cores)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            pos, crd = oracle.spgemm_assemble(A, B)
+            vals = oracle.spgemm_compute(A, B, pos, crd)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        kind = "port"
+    return (pos, crd, vals), list(ts), cores, kind
+
+
+# ---------------------------------------------------------------- GPU leg --
+
+class Timer:
+    """HIP events on torch's current stream, which the library launches on."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.e0 = torch.cuda.Event(enable_timing=True)
+        self.e1 = torch.cuda.Event(enable_timing=True)
+
+    def run(self, fn, steps, warmup):
+        torch = self.torch
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self.e0.record()
+        for _ in range(steps):
+            fn()
+        self.e1.record()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3
+        return self.e0.elapsed_time(self.e1) / steps, wall / steps
+
+
+def gpu_setup():
+    import torch
+    from taco_amd import _lib
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    L = _lib.lib()
+    _lib.check(L.taco_hip_set_stream(torch.cuda.current_stream().cuda_stream))
+    return torch, L, _lib
+
+
+def sched(_lib, algo, block=0, items=0, group=0, rows=0):
+    s = _lib.Schedule()
+    s.algo, s.block_size, s.items_per_thread = algo, block, items
+    s.group_size, s.rows_per_group = group, rows
+    return s
+
+
+def bench_spmv(args, A, x, name, workload_desc):
+    from taco_amd import workloads as W
+    y_ref, cpu_ts, cores, kind = (None, [], 0, None)
+    if not args.no_cpu:
+        log(f"cpu baseline: taco SpMV on {host_cores()} cores ...")
+        y_ref, cpu_ts, cores, kind = cpu_baseline_spmv(A, x, args.cpu_reps)
+    torch, L, _lib = gpu_setup()
+    from taco_amd import descriptors as D
+    pos = torch.from_numpy(A.pos).cuda()
+    crd = torch.from_numpy(A.crd).cuda()
+    vals = torch.from_numpy(A.vals).cuda()
+    xd = torch.from_numpy(x).cuda()
+    y = torch.empty(A.m, dtype=torch.float64, device="cuda")
+    a, xdsc, ydsc = D.csr(pos, crd, vals, A.shape), D.dense(xd), D.dense(y)
+    s_main = sched(_lib, _lib.MERGE_PATH, 256, args.items)
+    s_default = sched(_lib, _lib.THREAD_PER_ROW, 256)
+
+    def step(s):
+        return lambda: _lib.check(L.taco_hip_spmv_compute(ydsc.ptr, a.ptr, xdsc.ptr, s))
+
+    timer = Timer(torch)
+    log(f"gpu: {args.warmup} warmup + {args.steps} steps ...")
+    ms, wall_ms = timer.run(step(s_main), args.steps, args.warmup)
+    y_host = y.cpu().numpy()
+    ms_default, _ = timer.run(step(s_default), max(5, args.steps // 5), 2)
+    nbytes = W.algorithmic_bytes_spmv(A.m, A.n, A.nnz)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    check = None
+    if y_ref is not None:
+        check = float(np.abs(y_host - y_ref).max() / max(np.abs(y_ref).max(), 1e-300))
+    cpu = None
+    if cpu_ts:
+        cms = trimmed_mean(cpu_ts)
+        cpu = {"value": nbytes / (cms * 1e-3) / 1e9, "unit": "GB/s", "cores": cores,
+               "kind": kind, "ms": cms, "median_ms": float(np.median(cpu_ts)),
+               "sample": f"whole workload, {len(cpu_ts)} timed compute() calls "
+                         f"(10% trimmed mean), JIT excluded, inputs in host DRAM"}
+    return {
+        "metric": "SpMV HBM GB/s (% of 8 TB/s), SpMM GFLOP/s: learned vs default schedule at tol",
+        "value": gbs, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "wall_ms_per_step": wall_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seed-generated)",
+        "config": {"workload": name, "desc": workload_desc, "m": A.m, "n": A.n,
+                   "nnz": A.nnz, "schedule": f"merge_path items_per_thread={args.items}",
+                   "parallelism": "single GPU"},
+        "pct_of_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS,
+        "gflops": 2.0 * A.nnz / (ms * 1e-3) / 1e9,
+        "nnz_per_s": A.nnz / (ms * 1e-3),
+        "default_schedule": {"schedule": "thread_per_row (taco split(i,256))",
+                             "ms": ms_default, "speedup_vs_default": ms_default / ms},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_spmv_merge + k_spmv_merge_fixup (one compute call)",
+                     "algorithmic_bytes": nbytes},
+        "check_vs_cpu_reference": check,
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_spmm(args, A, dtype):
+    from taco_amd import workloads as W
+    K = 64
+    B = W.uniform(A.n * K, 2, W.STREAM_B).reshape(A.n, K)
+    C_ref, cpu_ts, cores, kind = (None, [], 0, None)
+    if not args.no_cpu and dtype == "f64":
+        log("cpu baseline: taco SpMM ...")
+        C_ref, cpu_ts, cores, kind = cpu_baseline_spmm(A, B, max(2, args.cpu_reps // 3))
+    torch, L, _lib = gpu_setup()
+    from taco_amd import descriptors as D
+    pos = torch.from_numpy(A.pos).cuda()
+    crd = torch.from_numpy(A.crd).cuda()
+    if dtype == "bf16":
+        vals = torch.from_numpy(A.vals.astype(np.float32)).to(torch.bfloat16).view(torch.int16).cuda()
+        Bd = torch.from_numpy(B.astype(np.float32)).to(torch.bfloat16).view(torch.int16).cuda()
+        C = torch.empty((A.m, K), dtype=torch.float32, device="cuda")
+        cdt = "f32"
+        sizes = (2, 2, 4)
+    else:
+        tdt = torch.float64 if dtype == "f64" else torch.float32
+        vals = torch.from_numpy(A.vals).to(tdt).cuda()
+        Bd = torch.from_numpy(B).to(tdt).cuda()
+        C = torch.empty((A.m, K), dtype=tdt, device="cuda")
+        cdt = dtype
+        sizes = (8, 8, 8) if dtype == "f64" else (4, 4, 4)
+    a = D.csr(pos, crd, vals, A.shape, dtype)
+    bdsc, cdsc = D.dense(Bd, dtype), D.dense(C, cdt)
+    s_main = sched(_lib, _lib.MFMA) if dtype == "bf16" and args.mfma else \
+        sched(_lib, _lib.MERGE_PATH, 256, args.spmm_items, 64)
+    s_default = sched(_lib, _lib.THREAD_PER_ROW, 256)
+
+    def step(s):
+        return lambda: _lib.check(L.taco_hip_spmm_compute(cdsc.ptr, a.ptr, bdsc.ptr, s))
+
+    timer = Timer(torch)
+    ms, wall_ms = timer.run(step(s_main), args.steps, args.warmup)
+    C_host = C.cpu().numpy().astype(np.float64)
+    ms_default, _ = timer.run(step(s_default), 2, 1)
+    nbytes = W.algorithmic_bytes_spmm(A.m, A.n, A.nnz, K, *sizes)
+    flops = 2.0 * A.nnz * K
+    gflops = flops / (ms * 1e-3) / 1e9
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    check = None
+    if C_ref is not None:
+        check = float(np.abs(C_host - C_ref).max() / max(np.abs(C_ref).max(), 1e-300))
+    cpu = None
+    if cpu_ts:
+        cms = trimmed_mean(cpu_ts)
+        cpu = {"value": flops / (cms * 1e-3) / 1e9, "unit": "GFLOP/s", "cores": cores,
+               "kind": kind, "ms": cms, "sample": f"whole workload, {len(cpu_ts)} calls"}
+    return {
+        "metric": "SpMM GFLOP/s", "value": gflops, "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "wall_ms_per_step": wall_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype if dtype != "bf16" else "bf16 (f32 accumulate)",
+        "data": "synthetic (seed-generated)",
+        "config": {"workload": f"spmm_powerlaw_2m_k64_{dtype}", "m": A.m, "nnz": A.nnz,
+                   "k": K, "schedule": "mfma" if (dtype == "bf16" and args.mfma) else "merge_path"},
+        "default_schedule": {"ms": ms_default, "speedup_vs_default": ms_default / ms},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": nbytes,
+                     "gather_inclusive_bytes": nbytes - sizes[1] * A.n * K + sizes[1] * A.nnz * K},
+        "check_vs_cpu_reference": check,
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_spgemm(args):
+    from taco_amd import workloads as W
+    log("generating 500k x 500k, 8 per row ...")
+    A, B = W.spgemm_500k()
+    ref, cpu_ts, cores, kind = (None, [], 0, None)
+    if not args.no_cpu:
+        log("cpu baseline: taco two-phase SpGEMM ...")
+        ref, cpu_ts, cores, kind = cpu_baseline_spgemm(A, B, max(2, args.cpu_reps // 3))
+    torch, L, _lib = gpu_setup()
+    import ctypes
+    from taco_amd import descriptors as D
+    ad = D.csr(torch.from_numpy(A.pos).cuda(), torch.from_numpy(A.crd).cuda(),
+               torch.from_numpy(A.vals).cuda(), A.shape)
+    bd = D.csr(torch.from_numpy(B.pos).cuda(), torch.from_numpy(B.crd).cuda(),
+               torch.from_numpy(B.vals).cuda(), B.shape)
+    outs = []
+
+    def assemble():
+        c = D.csr_empty((A.m, B.n))
+        _lib.check(L.taco_hip_spgemm_assemble(c.ptr, ad.ptr, bd.ptr))
+        outs.append(c)
+
+    def release():
+        for c in outs[:-1]:
+            for p in (c.level_ptr(1, 0), c.level_ptr(1, 1), c.vals_ptr()):
+                L.taco_hip_unified_free(ctypes.c_void_p(p))
+        del outs[:-1]
+
+    timer = Timer(torch)
+    ms_asm = []
+    for _ in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        assemble()
+        torch.cuda.synchronize()
+        ms_asm.append((time.perf_counter() - t0) * 1e3)
+        release()
+    ms_asm = float(np.median(ms_asm[args.warmup:]))
+    c = outs[-1]
+    ms_cmp, _ = timer.run(lambda: _lib.check(L.taco_hip_spgemm_compute(c.ptr, ad.ptr, bd.ptr)),
+                          args.steps, args.warmup)
+    nnzc = c.t.vals_size
+    check = None
+    if ref is not None:
+        pos = np.empty(A.m + 1, np.int32)
+        crd = np.empty(nnzc, np.int32)
+        vals = np.empty(nnzc)
+        for h, p in ((pos, c.level_ptr(1, 0)), (crd, c.level_ptr(1, 1)), (vals, c.vals_ptr())):
+            _lib.check(L.taco_hip_memcpy(h.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(p), h.nbytes, 1))
+        check = {"pos_equal": bool(np.array_equal(pos, ref[0])),
+                 "crd_equal": bool(np.array_equal(crd, ref[1])),
+                 "vals_max_rel": float(np.abs(vals - ref[2]).max() / np.abs(ref[2]).max())}
+    asm_bytes = 8 * (A.m + 1) + 4 * A.nnz + 4 * (B.m + 1) + 4 * B.nnz + 4 * nnzc
+    cmp_bytes = 12 * A.nnz + 12 * B.nnz + 12 * (A.m + 1) + 12 * nnzc
+    cpu = None
+    if cpu_ts:
+        cms = trimmed_mean(cpu_ts)
+        cpu = {"value": nnzc / (cms * 1e-3), "unit": "nnz(C)/s", "cores": cores,
+               "kind": kind, "ms": cms, "sample": "whole workload, assemble + compute"}
+    return {
+        "metric": "SpGEMM nnz(C)/s (assemble + compute)", "unit": "nnz(C)/s",
+        "value": nnzc / ((ms_asm + ms_cmp) * 1e-3), "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_asm + ms_cmp, "assemble_ms": ms_asm,
+        "compute_ms": ms_cmp, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seed-generated)",
+        "config": {"workload": "spgemm_500k_8perrow", "nnz_c": nnzc},
+        "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                     "achieved": cmp_bytes / (ms_cmp * 1e-3) / 1e9,
+                     "frac": cmp_bytes / (ms_cmp * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_spgemm_numeric",
+                     "assemble_gbs": asm_bytes / (ms_asm * 1e-3) / 1e9},
+        "check_vs_cpu_reference": check,
+        "cpu_baseline": cpu,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="spmv_powerlaw")
+    ap.add_argument("--items", type=int, default=8, help="SpMV merge items per thread")
+    ap.add_argument("--spmm-items", type=int, default=32)
+    ap.add_argument("--mfma", type=int, default=1)
+    ap.add_argument("--cpu-reps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU runs are out of scope for this backend)", file=sys.stderr)
+        sys.exit(2)
+    from taco_amd import workloads as W
+    w = args.workload
+    if w in ("spmv_powerlaw", "spmm_f64", "spmm_f32", "spmm_bf16"):
+        log("generating 2M x 2M power-law matrix ...")
+        t0 = time.perf_counter()
+        A = W.powerlaw_2m()
+        log(f"  nnz={A.nnz} in {time.perf_counter() - t0:.1f}s")
+        if w == "spmv_powerlaw":
+            x = W.uniform(A.n, 2, W.STREAM_X)
+            out = bench_spmv(args, A, x, "spmv_powerlaw_2m_f64",
+                             "2Mx2M CSR, Zipf(2) rows on [1,1e5] rescaled to mean 16")
+        else:
+            out = bench_spmm(args, A, w.split("_")[1])
+    elif w == "spmv_uniform":
+        A, x = W.spmv_uniform_100k()
+        out = bench_spmv(args, A, x, "spmv_uniform_100k_f64", "100kx100k, 10 per row")
+    elif w == "spgemm":
+        out = bench_spgemm(args)
+    else:
+        sys.exit(f"unknown workload {w}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

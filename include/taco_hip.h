@@ -1,0 +1,163 @@
+/* taco_hip: MI355X (gfx950, wave64) backend for taco's sparse hot path.
+ *
+ * This is the C ABI of libtaco_hip.so.  Every entry point takes plain
+ * pointers, sizes and taco_tensor_t descriptors; none takes a framework type.
+ * All functions returning int return 0 on success and a non-zero code on
+ * failure; taco_hip_last_error() then describes the failure (the reference
+ * calls exit() or ignores CUDA errors, src/codegen/codegen_cuda.cpp:58-67,
+ * src/cuda.cpp:45-51; this backend reports them instead).
+ *
+ * Execution model: kernels are enqueued on the library's current stream
+ * (taco_hip_set_stream; default: the null stream).  Entry points are
+ * stream-ordered; callers that need the reference's blocking behaviour
+ * (codegen_cuda.cpp:609 synchronises after every launch) call
+ * taco_hip_synchronize() or enable taco_hip_set_blocking(1).
+ */
+#ifndef TACO_HIP_H
+#define TACO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "taco/taco_tensor_t.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================
+ * Runtime: replaces taco's CUDA runtime (reference include/taco/cuda.h:15-28,
+ * src/cuda.cpp:11-72).
+ * ==================================================================== */
+
+/* should_use_CUDA_codegen (cuda.cpp:13): backend selector. */
+int taco_hip_should_use_codegen(void);
+/* set_CUDA_codegen_enabled (cuda.cpp:21-23). */
+void taco_hip_set_codegen_enabled(int enabled);
+/* should_use_CUDA_unified_memory (cuda.cpp:17-19). */
+int taco_hip_should_use_unified_memory(void);
+/* set_CUDA_unified_memory_enabled (cuda.cpp:25-28).  When disabled (the
+ * default), taco_hip_unified_alloc returns device-resident memory. */
+void taco_hip_set_unified_memory_enabled(int enabled);
+/* get_default_CUDA_compiler_flags (cuda.cpp:30-40): hipcc flags for the
+ * current device's architecture.  Returned string is owned by the library. */
+const char* taco_hip_get_default_compiler_flags(void);
+/* cuda_unified_alloc (cuda.cpp:54-64): NULL for size 0; managed memory when
+ * unified memory is enabled, device memory otherwise. */
+void* taco_hip_unified_alloc(size_t size);
+/* cuda_unified_free (cuda.cpp:66-72).  Frees memory from
+ * taco_hip_unified_alloc or any allocation this library hands out (outputs
+ * of assemble functions). */
+void taco_hip_unified_free(void* ptr);
+
+/* Description of the last failure on the calling thread. */
+const char* taco_hip_last_error(void);
+/* Number of visible HIP devices (0 on a machine without a GPU). */
+int taco_hip_device_count(void);
+/* gfx architecture name of the current device, e.g. "gfx950". */
+const char* taco_hip_device_arch(void);
+/* Stream used by all launches of this library on the calling thread. */
+int taco_hip_set_stream(void* stream);
+void* taco_hip_get_stream(void);
+/* 1: synchronise the stream at the end of every entry point. */
+void taco_hip_set_blocking(int blocking);
+int taco_hip_synchronize(void);
+/* kind: 0 host->device, 1 device->host, 2 device->device. Synchronous. */
+int taco_hip_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int taco_hip_memset(void* dst, int value, size_t bytes);
+
+/* ======================================================================
+ * Seeded synthetic inputs (host, OpenMP).  Counter-based (splitmix64), so a
+ * given (seed, row) always yields the same row regardless of thread count.
+ * ==================================================================== */
+
+/* Uniform values in [lo, hi) for element indices [0, n) of stream `stream`. */
+void taco_hip_gen_uniform_f64(int64_t n, uint64_t seed, uint64_t stream,
+                              double lo, double hi, double* out);
+/* Per-row entry counts: Zipf(alpha) on [kmin, kmax] drawn by inverse CDF,
+ * rescaled so the mean is `mean` (round to nearest, clip to [kmin, kmax]).
+ * mean <= 0 disables the rescale.  Returns the total. */
+int64_t taco_hip_gen_row_counts_zipf(int32_t m, double alpha, int32_t kmin,
+                                     int32_t kmax, double mean, uint64_t seed,
+                                     int32_t* counts);
+/* Per-row counts uniform in [kmin, kmax]. Returns the total. */
+int64_t taco_hip_gen_row_counts_uniform(int32_t m, int32_t kmin, int32_t kmax,
+                                        uint64_t seed, int32_t* counts);
+/* pos[0] = 0, pos[i+1] = pos[i] + counts[i].  Returns 0, or 1 when the total
+ * exceeds INT32_MAX (taco's int32 index arithmetic). */
+int taco_hip_counts_to_pos(int32_t m, const int32_t* counts, int32_t* pos);
+/* Column indices of a CSR pattern: row i gets pos[i+1]-pos[i] distinct
+ * columns in [0, n).  structure 0: uniform without replacement; 1: banded,
+ * uniform without replacement from [i-band, i+band] clipped to [0, n);
+ * 2: block diagonal, dense `band` x `band` blocks (row count ignored beyond
+ * the block width).  Each row is sorted ascending.  Returns 0, or 1 if a row
+ * asks for more columns than its structure holds. */
+int taco_hip_gen_csr_columns(int32_t m, int32_t n, const int32_t* pos,
+                             int32_t structure, int32_t band, uint64_t seed,
+                             int32_t* crd);
+
+/* ======================================================================
+ * Hand-written HIP kernels for the hot path.  Signatures follow the
+ * generated-module ABI (int f(taco_tensor_t* results..., operands...),
+ * reference src/codegen/codegen.cpp:514-535): the result first, then the
+ * operands.  Value type follows the descriptors' csize.
+ * ==================================================================== */
+
+typedef enum taco_hip_algo {
+  /* One thread per row: taco's default GPU schedule split(i,256) ->
+   * GPUBlock/GPUThread (reference transformations.cpp:1585-1596). */
+  TACO_HIP_THREAD_PER_ROW = 0,
+  /* group_size lanes per row, lanes stride the row's entries, wave-level
+   * reduction: scheduleSpMVRowsGPU (tests-scheduling-eval.cpp:211-223);
+   * for SpMM the lanes run along k instead (scheduleSpMMRowsGPU, :345-355). */
+  TACO_HIP_GROUP_PER_ROW = 1,
+  /* Equal entry chunks per thread/wave, row boundaries resolved with
+   * segmented wave reductions and atomics: scheduleSpMVGPU
+   * (tests-scheduling-eval.cpp:193-209), scheduleSpMMGPU (:249-268). */
+  TACO_HIP_NNZ_SPLIT = 2,
+  /* Merge-path over (row ends, entries): every thread gets the same number
+   * of merge items; cross-workgroup rows are fixed up from a carry array in
+   * order, without atomics.  No reference counterpart. */
+  TACO_HIP_MERGE_PATH = 3,
+  /* SpMM only: bf16 operands on MFMA, f32 accumulate. */
+  TACO_HIP_MFMA = 4
+} taco_hip_algo;
+
+typedef struct taco_hip_schedule {
+  int32_t algo;             /* taco_hip_algo */
+  int32_t block_size;       /* threads per workgroup, multiple of 64 */
+  int32_t items_per_thread; /* entries (NNZ_SPLIT) or merge items (MERGE_PATH) */
+  int32_t group_size;       /* lanes per row (GROUP_PER_ROW), power of 2 <= 64 */
+  int32_t rows_per_group;   /* rows per lane group (GROUP_PER_ROW) */
+  int32_t reserved[3];
+} taco_hip_schedule;
+
+/* Default schedule per operation kind (0 SpMV, 1 SpMM) for a matrix. */
+void taco_hip_default_schedule(int32_t op, int32_t m, int64_t nnz, int32_t k,
+                               taco_hip_schedule* out);
+
+/* y(i) = A(i,j) * x(j).  A: CSR {dense, compressed}; x, y: dense vectors.
+ * f64 and f32.  sched == NULL uses taco_hip_default_schedule. */
+int taco_hip_spmv_compute(taco_tensor_t* y, taco_tensor_t* A,
+                          taco_tensor_t* x, const taco_hip_schedule* sched);
+
+/* C(i,k) = A(i,j) * B(j,k).  A CSR; B, C dense row-major.  f64, f32, and
+ * bf16 A/B with f32 C (A->csize == B->csize == 16, C->csize == 32). */
+int taco_hip_spmm_compute(taco_tensor_t* C, taco_tensor_t* A,
+                          taco_tensor_t* B, const taco_hip_schedule* sched);
+
+/* Two-phase SpGEMM C(i,k) = A(i,j) * B(j,k), all CSR, f64 values.
+ * assemble: computes the structural pattern (sorted columns, bit-exact with
+ * taco's CPU assemble) and allocates C's pos, crd and vals (freed with
+ * taco_hip_unified_free), storing them into C->indices[1] and C->vals.
+ * compute: fills C->vals on that pattern. */
+int taco_hip_spgemm_assemble(taco_tensor_t* C, taco_tensor_t* A,
+                             taco_tensor_t* B);
+int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,
+                            taco_tensor_t* B);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,120 @@
+"""ctypes binding of libtaco_hip.so (the C ABI declared in include/taco_hip.h).
+
+The library is built in-tree by ``taco_amd.build.build()``.  There is no
+fallback: if the shared object is missing or does not load, every entry
+point raises, so a run can never silently compute on the CPU.
+"""
+import ctypes
+import os
+from ctypes import (POINTER, Structure, c_char_p, c_double, c_int, c_int32,
+                    c_int64, c_size_t, c_uint8, c_uint64, c_void_p)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtaco_hip.so")
+
+
+class TacoTensorT(Structure):
+    """Mirror of ``taco_tensor_t`` (include/taco/taco_tensor_t.h)."""
+    _fields_ = [
+        ("order", c_int32),
+        ("dimensions", POINTER(c_int32)),
+        ("csize", c_int32),
+        ("mode_ordering", POINTER(c_int32)),
+        ("mode_types", POINTER(c_int)),
+        ("indices", POINTER(POINTER(POINTER(c_uint8)))),
+        ("vals", POINTER(c_uint8)),
+        ("fill_value", POINTER(c_uint8)),
+        ("vals_size", c_int32),
+    ]
+
+
+class Schedule(Structure):
+    """Mirror of ``taco_hip_schedule``."""
+    _fields_ = [
+        ("algo", c_int32),
+        ("block_size", c_int32),
+        ("items_per_thread", c_int32),
+        ("group_size", c_int32),
+        ("rows_per_group", c_int32),
+        ("reserved", c_int32 * 3),
+    ]
+
+
+# taco_hip_algo
+THREAD_PER_ROW, GROUP_PER_ROW, NNZ_SPLIT, MERGE_PATH, MFMA = range(5)
+ALGO_NAMES = {THREAD_PER_ROW: "thread_per_row", GROUP_PER_ROW: "group_per_row",
+              NNZ_SPLIT: "nnz_split", MERGE_PATH: "merge_path", MFMA: "mfma"}
+
+P_T = POINTER(TacoTensorT)
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "taco_hip_should_use_codegen": (c_int, []),
+    "taco_hip_set_codegen_enabled": (None, [c_int]),
+    "taco_hip_should_use_unified_memory": (c_int, []),
+    "taco_hip_set_unified_memory_enabled": (None, [c_int]),
+    "taco_hip_get_default_compiler_flags": (c_char_p, []),
+    "taco_hip_unified_alloc": (c_void_p, [c_size_t]),
+    "taco_hip_unified_free": (None, [c_void_p]),
+    "taco_hip_last_error": (c_char_p, []),
+    "taco_hip_device_count": (c_int, []),
+    "taco_hip_device_arch": (c_char_p, []),
+    "taco_hip_set_stream": (c_int, [c_void_p]),
+    "taco_hip_get_stream": (c_void_p, []),
+    "taco_hip_set_blocking": (None, [c_int]),
+    "taco_hip_synchronize": (c_int, []),
+    "taco_hip_memcpy": (c_int, [c_void_p, c_void_p, c_size_t, c_int]),
+    "taco_hip_memset": (c_int, [c_void_p, c_int, c_size_t]),
+    "taco_hip_gen_uniform_f64": (None, [c_int64, c_uint64, c_uint64, c_double,
+                                        c_double, c_void_p]),
+    "taco_hip_gen_row_counts_zipf": (c_int64, [c_int32, c_double, c_int32,
+                                               c_int32, c_double, c_uint64,
+                                               c_void_p]),
+    "taco_hip_gen_row_counts_uniform": (c_int64, [c_int32, c_int32, c_int32,
+                                                  c_uint64, c_void_p]),
+    "taco_hip_counts_to_pos": (c_int, [c_int32, c_void_p, c_void_p]),
+    "taco_hip_gen_csr_columns": (c_int, [c_int32, c_int32, c_void_p, c_int32,
+                                         c_int32, c_uint64, c_void_p]),
+    "taco_hip_default_schedule": (None, [c_int32, c_int32, c_int64, c_int32,
+                                         POINTER(Schedule)]),
+    "taco_hip_spmv_compute": (c_int, [P_T, P_T, P_T, POINTER(Schedule)]),
+    "taco_hip_spmm_compute": (c_int, [P_T, P_T, P_T, POINTER(Schedule)]),
+    "taco_hip_spgemm_assemble": (c_int, [P_T, P_T, P_T]),
+    "taco_hip_spgemm_compute": (c_int, [P_T, P_T, P_T]),
+    "init_taco_tensor_t": (P_T, [c_int32, c_int32, POINTER(c_int32),
+                                 POINTER(c_int32), POINTER(c_int), c_void_p]),
+    "deinit_taco_tensor_t": (None, [P_T]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+class TacoHipError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library; raises if it was not built or does not load."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TacoHipError(
+            f"{LIB_PATH} is missing: build it with taco_amd.build.build() "
+            "(make -C taco_amd/csrc)")
+    handle = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = handle
+    return _lib
+
+
+def check(rc, what="taco_hip call"):
+    if rc != 0:
+        msg = lib().taco_hip_last_error().decode(errors="replace")
+        raise TacoHipError(f"{what} failed ({rc}): {msg}")
+    return rc

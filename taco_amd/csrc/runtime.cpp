@@ -1,0 +1,318 @@
+// HIP runtime of the backend: the function set of taco's CUDA runtime
+// (reference include/taco/cuda.h:15-28, src/cuda.cpp:11-72) plus the
+// descriptor allocation of src/taco_tensor_t.cpp, written for one MI355X with
+// device-resident memory by default and managed memory as an option.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace taco_hip {
+
+namespace {
+
+thread_local std::string t_last_error;
+thread_local hipStream_t t_stream = nullptr;
+thread_local bool t_blocking = false;
+
+// Codegen is on by default when a device is present (taco enables its CUDA
+// codegen whenever CUDA is built, cuda.cpp:11-12).
+int g_codegen_enabled = -1;
+bool g_unified_memory = false;
+
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+constexpr int kSlots = 4;
+constexpr int kMaxDevices = 16;
+std::mutex g_ws_mutex;
+Workspace g_ws[kMaxDevices][kSlots];
+
+struct WorkspaceReaper {
+  ~WorkspaceReaper() {
+    // Process exit: the runtime may already be tearing down, so errors are
+    // deliberately ignored here.
+    for (auto& dev : g_ws)
+      for (auto& w : dev)
+        if (w.ptr) (void)hipFree(w.ptr);
+  }
+} g_reaper;
+
+std::string g_flags;
+std::string g_arch;
+
+}  // namespace
+
+int fail(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  t_last_error = buf;
+  return 1;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  char buf[1024];
+  snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e,
+           hipGetErrorString(e), what, file, line);
+  t_last_error = buf;
+  return (int)e != 0 ? (int)e : 1;
+}
+
+hipStream_t current_stream() { return t_stream; }
+bool blocking() { return t_blocking; }
+
+int finish(const char* what) {
+  TH_LAUNCH_CHECK(what);
+  if (t_blocking) TH_CHECK(hipStreamSynchronize(t_stream));
+  return 0;
+}
+
+void* workspace(size_t bytes, int slot) {
+  if (slot < 0 || slot >= kSlots) {
+    fail("workspace slot %d out of range", slot);
+    return nullptr;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    fail("workspace: no current device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  Workspace& w = g_ws[dev][slot];
+  if (w.bytes >= bytes && w.ptr) return w.ptr;
+  if (w.ptr) {
+    // The old buffer may still be in use by queued work on any stream.
+    if (hipDeviceSynchronize() != hipSuccess || hipFree(w.ptr) != hipSuccess) {
+      fail("workspace: failed to release old buffer");
+      return nullptr;
+    }
+    w.ptr = nullptr;
+    w.bytes = 0;
+  }
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  want = (want + 4095) & ~size_t(4095);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    hip_fail(e, "hipMalloc(workspace)", __FILE__, __LINE__);
+    return nullptr;
+  }
+  w.ptr = p;
+  w.bytes = want;
+  return p;
+}
+
+int view_csr(const taco_tensor_t* t, const char* name, CsrView* out) {
+  TH_REQUIRE(t, "%s: null descriptor", name);
+  TH_REQUIRE(t->order == 2, "%s: expected a matrix, order is %d", name,
+             t->order);
+  TH_REQUIRE(t->mode_types && t->mode_types[0] == taco_mode_dense &&
+                 t->mode_types[1] == taco_mode_sparse,
+             "%s: expected CSR levels {dense, compressed}", name);
+  TH_REQUIRE(t->mode_ordering && t->mode_ordering[0] == 0 &&
+                 t->mode_ordering[1] == 1,
+             "%s: expected mode ordering {0,1}", name);
+  TH_REQUIRE(t->indices && t->indices[1], "%s: missing index arrays", name);
+  out->m = t->dimensions[0];
+  out->n = t->dimensions[1];
+  TH_REQUIRE(out->m >= 0 && out->n >= 0, "%s: negative dimension", name);
+  out->pos = (const int32_t*)t->indices[1][0];
+  out->crd = (const int32_t*)t->indices[1][1];
+  out->vals = t->vals;
+  out->vt = vtype_of(t->csize);
+  TH_REQUIRE(out->vt != VType::Invalid, "%s: unsupported csize %d", name,
+             t->csize);
+  TH_REQUIRE(out->pos != nullptr || out->m == 0, "%s: null pos", name);
+  if (t->vals_size > 0) {
+    out->nnz = t->vals_size;
+  } else if (out->m == 0) {
+    out->nnz = 0;
+  } else {
+    int32_t last = 0;
+    TH_CHECK(hipMemcpy(&last, out->pos + out->m, sizeof(int32_t),
+                       hipMemcpyDeviceToHost));
+    out->nnz = last;
+  }
+  TH_REQUIRE(out->nnz >= 0, "%s: negative nnz", name);
+  TH_REQUIRE(out->nnz == 0 || (out->crd && out->vals),
+             "%s: null crd/vals with nnz %lld", name, (long long)out->nnz);
+  return 0;
+}
+
+int view_dense(const taco_tensor_t* t, const char* name, int order,
+               DenseView* out) {
+  TH_REQUIRE(t, "%s: null descriptor", name);
+  TH_REQUIRE(t->order == order, "%s: expected order %d, got %d", name, order,
+             t->order);
+  for (int k = 0; k < order; ++k) {
+    TH_REQUIRE(t->mode_types[k] == taco_mode_dense, "%s: level %d not dense",
+               name, k);
+    TH_REQUIRE(t->mode_ordering[k] == k,
+               "%s: only row-major (identity) ordering is supported", name);
+    out->dims[k] = t->dimensions[k];
+    TH_REQUIRE(out->dims[k] >= 0, "%s: negative dimension", name);
+  }
+  out->order = order;
+  out->vals = t->vals;
+  out->vt = vtype_of(t->csize);
+  TH_REQUIRE(out->vt != VType::Invalid, "%s: unsupported csize %d", name,
+             t->csize);
+  return 0;
+}
+
+}  // namespace taco_hip
+
+using namespace taco_hip;
+
+extern "C" {
+
+int taco_hip_should_use_codegen(void) {
+  if (g_codegen_enabled < 0) g_codegen_enabled = taco_hip_device_count() > 0;
+  return g_codegen_enabled;
+}
+
+void taco_hip_set_codegen_enabled(int enabled) {
+  g_codegen_enabled = enabled ? 1 : 0;
+}
+
+int taco_hip_should_use_unified_memory(void) { return g_unified_memory; }
+
+void taco_hip_set_unified_memory_enabled(int enabled) {
+  g_unified_memory = enabled != 0;
+}
+
+const char* taco_hip_device_arch(void) {
+  if (!g_arch.empty()) return g_arch.c_str();
+  hipDeviceProp_t prop;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    return "gfx950";  // the only architecture this backend targets
+  }
+  std::string name = prop.gcnArchName;
+  size_t colon = name.find(':');
+  g_arch = name.substr(0, colon);
+  return g_arch.c_str();
+}
+
+const char* taco_hip_get_default_compiler_flags(void) {
+  // Counterpart of nvcc's "-w -O3 -Xcompiler -fPIC -shared -ffast-math -O3
+  // --generate-code arch=compute_XY,code=sm_XY" (cuda.cpp:30-40).  Packed
+  // FP32 instructions are disabled: hipcc forms them from plain float code.
+  g_flags = std::string("-w -O3 -fPIC -shared -std=c++17 --offload-arch=") +
+            taco_hip_device_arch() +
+            " -Xclang -target-feature -Xclang -packed-fp32-ops"
+            " -munsafe-fp-atomics";
+  return g_flags.c_str();
+}
+
+void* taco_hip_unified_alloc(size_t size) {
+  if (size == 0) return nullptr;
+  void* p = nullptr;
+  hipError_t e = g_unified_memory ? hipMallocManaged(&p, size)
+                                  : hipMalloc(&p, size);
+  if (e != hipSuccess) {
+    hip_fail(e, "taco_hip_unified_alloc", __FILE__, __LINE__);
+    return nullptr;
+  }
+  return p;
+}
+
+void taco_hip_unified_free(void* ptr) {
+  if (!ptr) return;
+  hipError_t e = hipFree(ptr);
+  if (e != hipSuccess) hip_fail(e, "taco_hip_unified_free", __FILE__, __LINE__);
+}
+
+const char* taco_hip_last_error(void) { return t_last_error.c_str(); }
+
+int taco_hip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int taco_hip_set_stream(void* stream) {
+  t_stream = (hipStream_t)stream;
+  return 0;
+}
+
+void* taco_hip_get_stream(void) { return (void*)t_stream; }
+
+void taco_hip_set_blocking(int b) { t_blocking = b != 0; }
+
+int taco_hip_synchronize(void) {
+  TH_CHECK(hipStreamSynchronize(t_stream));
+  return 0;
+}
+
+int taco_hip_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  if (bytes == 0) return 0;
+  hipMemcpyKind k = kind == 0   ? hipMemcpyHostToDevice
+                    : kind == 1 ? hipMemcpyDeviceToHost
+                                : hipMemcpyDeviceToDevice;
+  TH_CHECK(hipMemcpyAsync(dst, src, bytes, k, t_stream));
+  TH_CHECK(hipStreamSynchronize(t_stream));
+  return 0;
+}
+
+int taco_hip_memset(void* dst, int value, size_t bytes) {
+  if (bytes == 0) return 0;
+  TH_CHECK(hipMemsetAsync(dst, value, bytes, t_stream));
+  TH_CHECK(hipStreamSynchronize(t_stream));
+  return 0;
+}
+
+// ---- descriptor allocation (reference src/taco_tensor_t.cpp:32-79) ----
+// Descriptors are host structures on this backend: kernels receive raw
+// device pointers unpacked on the host, so the descriptor never needs to be
+// device-visible.
+
+taco_tensor_t* init_taco_tensor_t(int32_t order, int32_t csize,
+                                  int32_t* dimensions, int32_t* modeOrdering,
+                                  taco_mode_t* mode_types, void* fill_ptr) {
+  taco_tensor_t* t = (taco_tensor_t*)calloc(1, sizeof(taco_tensor_t));
+  if (!t) return nullptr;
+  t->order = order;
+  t->csize = csize;
+  size_t n = order > 0 ? (size_t)order : 1;
+  t->dimensions = (int32_t*)calloc(n, sizeof(int32_t));
+  t->mode_ordering = (int32_t*)calloc(n, sizeof(int32_t));
+  t->mode_types = (taco_mode_t*)calloc(n, sizeof(taco_mode_t));
+  t->indices = (uint8_t***)calloc(n, sizeof(uint8_t**));
+  int fill_bytes = csize / 8 > 0 ? csize / 8 : 1;
+  t->fill_value = (uint8_t*)calloc((size_t)fill_bytes, 1);
+  if (fill_ptr) memcpy(t->fill_value, fill_ptr, (size_t)(csize / 8));
+  for (int32_t i = 0; i < order; i++) {
+    t->dimensions[i] = dimensions[i];
+    t->mode_ordering[i] = modeOrdering[i];
+    t->mode_types[i] = mode_types[i];
+    t->indices[i] = (uint8_t**)calloc(
+        t->mode_types[i] == taco_mode_sparse ? 2 : 1, sizeof(uint8_t*));
+  }
+  return t;
+}
+
+void deinit_taco_tensor_t(taco_tensor_t* t) {
+  if (!t) return;
+  for (int i = 0; i < t->order; i++) free(t->indices[i]);
+  free(t->indices);
+  free(t->dimensions);
+  free(t->mode_ordering);
+  free(t->mode_types);
+  free(t->fill_value);
+  free(t);
+}
+
+}  // extern "C"
