@@ -260,6 +260,9 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(
   }
   // Long row: LDS slot per pattern entry (nc <= kHashCap was checked by the
   // symbolic pass), candidates added one at a time in encounter order.
+  // A caller-supplied pattern longer than the slot array is left unwritten
+  // rather than overrunning LDS.
+  if (nc > kHashCap) return;
   double* acc = s_acc[wave];
   for (int q = lane; q < nc; q += kWave) acc[q] = 0;
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");

@@ -191,7 +191,7 @@ def test_spmm_mfma_integer_exact():
 
 def spgemm_cases():
     yield "fixed8", W.csr_fixed(4000, 3500, 8, 21), W.csr_fixed(3500, 3000, 8, 22)
-    yield "long_rows", W.csr_powerlaw(1500, 1200, seed=23, kmax=400), \
+    yield "long_rows", W.csr_powerlaw(1500, 1200, seed=23, kmax=300), \
         W.csr_fixed(1200, 2000, 3, 24)
     yield "empty", W.Csr(4, 3, np.zeros(5, np.int32), np.zeros(0, np.int32),
                          np.zeros(0)), W.csr_fixed(3, 6, 2, 25)
