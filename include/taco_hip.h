@@ -156,6 +156,91 @@ int taco_hip_spgemm_assemble(taco_tensor_t* C, taco_tensor_t* A,
 int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,
                             taco_tensor_t* B);
 
+/* Device exclusive scan: pos[0] = 0, pos[i+1] = pos[i] + counts[i], on the
+ * current stream.  Returns non-zero (and records an error) if the total
+ * exceeds the int32 index range; synchronises the stream to check. */
+int taco_hip_device_exclusive_scan(const int32_t* counts, int32_t m,
+                                   int32_t* pos);
+/* A device buffer of at least `bytes` from the library's reusable workspace
+ * (slots 0..7; slots 4..7 are reserved for generated modules).  Valid until
+ * the next request of the same slot on this device; never freed by the
+ * caller.  NULL on failure. */
+void* taco_hip_workspace_get(size_t bytes, int slot);
+
+/* ======================================================================
+ * Code generation (CodeGen_HIP) and the JIT module loader.
+ * ==================================================================== */
+
+/* Statement: 0 SpMV y(i)=A(i,j)*x(j); 1 SpMM C(i,k)=A(i,j)*B(j,k) with dense
+ * row-major B, C; 2 SpGEMM C(i,k)=A(i,j)*B(j,k) with CSR C (two-phase). */
+typedef enum taco_hip_gen_op {
+  TACO_HIP_OP_SPMV = 0,
+  TACO_HIP_OP_SPMM = 1,
+  TACO_HIP_OP_SPGEMM = 2
+} taco_hip_gen_op;
+
+/* Schedule families, each the lowering of one of the reference's GPU
+ * schedules (test/tests-scheduling-eval.cpp):
+ *   DEFAULT    parallelizeOuterLoop's split(i,block) -> GPUBlock/GPUThread
+ *              (src/index_notation/transformations.cpp:1585-1596)
+ *   WARP_ROWS  scheduleSpMVRowsGPU (:211-223): rows over lane groups,
+ *              entries over the lanes of a group, GPUWarpReduction
+ *   NNZ_SPLIT  scheduleSpMVGPU (:193-209) / scheduleSpMMGPU (:249-268):
+ *              fuse(i,j) -> pos -> split into block/warp/thread entry
+ *              chunks, block-start search, atomics
+ *   TWO_PHASE  SpGEMM assemble (pattern) + compute (values), one wave per
+ *              row (taco's CPU two-phase SpGEMM, tests-scheduling-eval.cpp:
+ *              68-94, made GPU-parallel) */
+typedef enum taco_hip_gen_family {
+  TACO_HIP_GEN_DEFAULT = 0,
+  TACO_HIP_GEN_WARP_ROWS = 1,
+  TACO_HIP_GEN_NNZ_SPLIT = 2,
+  TACO_HIP_GEN_TWO_PHASE = 3
+} taco_hip_gen_family;
+
+typedef struct taco_hip_gen_spec {
+  int32_t op;             /* taco_hip_gen_op */
+  int32_t family;         /* taco_hip_gen_family */
+  int32_t csize;          /* value bits: 64 or 32 */
+  int32_t block_size;     /* BLOCK_SIZE (threads per workgroup) */
+  int32_t nnz_per_thread; /* NNZ_PER_THREAD (SpMV) / NNZ_PER_WARP (SpMM) */
+  int32_t warp_size;      /* lanes per group (the reference's WARP_SIZE) */
+  int32_t rows_per_warp;  /* ROWS_PER_WARP (WARP_ROWS) */
+  int32_t bound;          /* bound(dense_val, B) of scheduleSpMMGPU: K <= B*warp */
+} taco_hip_gen_spec;
+
+/* Lower `spec` and print it.  what = 0: HIP module source (CodeGen_HIP);
+ * what = 1: the IR.  Returns a malloc'ed string (free with
+ * taco_hip_free_string) or NULL with the error recorded. */
+char* taco_hip_generate(const taco_hip_gen_spec* spec, int32_t what);
+void taco_hip_free_string(char* s);
+
+/* JIT module (reference include/taco/codegen/module.h:19-58,
+ * src/codegen/module.cpp:111-215): source text -> hipcc -> shared object ->
+ * dlopen; functions are called through their _shim_ wrappers.  Compiled
+ * objects are cached by a hash of (source, compiler, flags) in
+ * $TACO_HIP_CACHE, default <library dir>/_jit_cache.  The compiler is
+ * $TACO_HIPCC (default /opt/rocm/bin/hipcc), flags $TACO_HIPCCFLAGS
+ * (default taco_hip_get_default_compiler_flags()). */
+typedef struct taco_hip_module taco_hip_module;
+taco_hip_module* taco_hip_module_create(void);
+int taco_hip_module_set_source(taco_hip_module* m, const char* source);
+const char* taco_hip_module_get_source(const taco_hip_module* m);
+/* Compiles (or finds in the cache) and loads; returns 0 on success. */
+int taco_hip_module_compile(taco_hip_module* m);
+/* Path of the loaded shared object ("" before compile). */
+const char* taco_hip_module_path(const taco_hip_module* m);
+void* taco_hip_module_get_func(taco_hip_module* m, const char* name);
+/* callFuncPacked: calls _shim_<name>(args). */
+int taco_hip_module_call_packed(taco_hip_module* m, const char* name,
+                                void** args);
+void taco_hip_module_destroy(taco_hip_module* m);
+
+/* Used by generated modules: error reporting and end-of-call handling. */
+int taco_hip_module_report(int code, const char* what);
+int taco_hip_module_finish(int rc);
+int taco_hip_host_read_bytes(void* dst, const void* src, size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,431 @@
+/* Runtime helpers for modules emitted by CodeGen_HIP (and shared with the
+ * hand-written kernels).  The counterpart of the helper prelude taco prints
+ * into every CUDA module (reference src/codegen/codegen_cuda.cpp:30-159),
+ * written for gfx950 wave64: 64-bit ballots, lane groups of any power of two
+ * up to 64, hardware float atomics, and errors reported to the caller instead
+ * of exit().
+ *
+ * HIP C++ only (included by hipcc-compiled sources). */
+#ifndef TACO_HIP_RT_H
+#define TACO_HIP_RT_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <limits.h>
+
+#include "taco/taco_tensor_t.h"
+
+extern "C" {
+void* taco_hip_get_stream(void);
+void* taco_hip_unified_alloc(size_t size);
+void taco_hip_unified_free(void* ptr);
+int taco_hip_module_finish(int rc);
+int taco_hip_module_report(int code, const char* what);
+int taco_hip_host_read_bytes(void* dst, const void* src, size_t bytes);
+int taco_hip_device_exclusive_scan(const int32_t* counts, int32_t m,
+                                   int32_t* pos);
+void* taco_hip_workspace_get(size_t bytes, int slot);
+}
+
+#define TACO_HIP_CHECK(call)                                               \
+  do {                                                                     \
+    int taco_rc_ = (int)(call);                                            \
+    if (taco_rc_ != 0) return taco_hip_module_report(taco_rc_, #call);     \
+  } while (0)
+#define TACO_HIP_MIN(a, b) ((a) < (b) ? (a) : (b))
+#define TACO_HIP_MAX(a, b) ((a) > (b) ? (a) : (b))
+
+static inline hipStream_t taco_hip_stream() {
+  return (hipStream_t)taco_hip_get_stream();
+}
+
+/* Host read of one element of a device array (synchronous).  Used where
+ * taco's generated host code reads an index array directly, which it can do
+ * only under unified memory. */
+template <typename T>
+static inline T taco_hip_host_read(const T* p, int64_t i) {
+  T v = T();
+  (void)taco_hip_host_read_bytes(&v, p + i, sizeof(T));
+  return v;
+}
+
+/* Reports a non-zero status word written by a row kernel (synchronous). */
+static inline int taco_hip_check_status(const int32_t* status) {
+  int32_t v = taco_hip_host_read(status, 0);
+  if (v & 1)
+    return taco_hip_module_report(3, "a row exceeds the distinct-column "
+                                     "capacity of the row kernel");
+  if (v & 2)
+    return taco_hip_module_report(4, "result exceeds the int32 index range");
+  return 0;
+}
+
+/* ---- index searches (semantics of codegen_cuda.cpp:68-141) ---- */
+
+/* Smallest idx in [lo, hi] with array[idx] >= target (hi if none below). */
+__device__ __host__ inline int32_t taco_hip_binarySearchAfter(
+    const int32_t* array, int32_t lo, int32_t hi, int32_t target) {
+  if (array[lo] >= target) return lo;
+  while (hi - lo > 1) { /* array[lo] < target <= array[hi] */
+    int32_t mid = lo + (hi - lo) / 2;
+    if (array[mid] < target) lo = mid; else hi = mid;
+  }
+  return hi;
+}
+
+/* Largest idx in [lo, hi] with array[idx] <= target. */
+__device__ __host__ inline int32_t taco_hip_binarySearchBefore(
+    const int32_t* array, int32_t lo, int32_t hi, int32_t target) {
+  if (array[hi] <= target) return hi;
+  while (hi - lo > 1) { /* array[lo] <= target < array[hi] */
+    int32_t mid = lo + (hi - lo) / 2;
+    if (array[mid] <= target) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+/* results[b] = binarySearchBefore(array, lo, hi, b * per_block) for b in
+ * [0, num_blocks]: the row where each block's entry chunk starts. */
+__global__ __launch_bounds__(256) void taco_hip_binarySearchBeforeBlock(
+    const int32_t* __restrict__ array, int32_t* __restrict__ results,
+    int32_t lo, int32_t hi, int32_t per_block, int32_t num_blocks) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > num_blocks) return;
+  int64_t target = b * (int64_t)per_block;
+  results[b] = taco_hip_binarySearchBefore(
+      array, lo, hi, target > INT_MAX ? INT_MAX : (int32_t)target);
+}
+
+static inline int taco_hip_binarySearchBeforeBlockLaunch(
+    const int32_t* array, int32_t* results, int32_t lo, int32_t hi,
+    int32_t per_block, int32_t block_size, int32_t num_blocks) {
+  int64_t grid = ((int64_t)num_blocks + 1 + block_size - 1) / block_size;
+  if (grid <= 0) return 0;
+  hipLaunchKernelGGL(taco_hip_binarySearchBeforeBlock, dim3((unsigned)grid),
+                     dim3(256), 0, taco_hip_stream(), array, results, lo, hi,
+                     per_block, num_blocks);
+  return (int)hipGetLastError();
+}
+
+/* ---- lane-group reductions and atomics ---- */
+
+template <int G, typename T>
+__device__ inline T taco_hip_group_sum(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+__device__ inline void taco_hip_atomicAdd(double* a, int64_t i, double v) {
+  unsafeAtomicAdd(a + i, v);
+}
+__device__ inline void taco_hip_atomicAdd(float* a, int64_t i, float v) {
+  unsafeAtomicAdd(a + i, v);
+}
+__device__ inline void taco_hip_atomicAdd(int32_t* a, int64_t i, int32_t v) {
+  atomicAdd(a + i, v);
+}
+
+/* Wave64 form of atomicAddWarp (codegen_cuda.cpp:142-159): when the whole
+ * wave is active and every lane targets the first lane's index, the wave
+ * sums with a shuffle tree and one lane issues a single atomic; otherwise
+ * each lane issues its own. */
+template <typename T>
+__device__ inline void taco_hip_atomicAddWave(T* a, int64_t i, T v) {
+  const uint64_t active = __ballot(1);
+  const int64_t lead = __shfl(i, 0, 64);
+  const bool same = __ballot(i == lead) == active;
+  if (active == ~0ull && same) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (__lane_id() == 0) taco_hip_atomicAdd(a, i, v);
+  } else {
+    taco_hip_atomicAdd(a, i, v);
+  }
+}
+
+/* ---- fills (a kernel rather than a memset node, safe under capture) ---- */
+
+template <typename T>
+__global__ __launch_bounds__(256) void taco_hip_fill_kernel(T* p, int64_t n,
+                                                            T v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+template <typename T>
+static inline int taco_hip_fill(T* p, int64_t n, T v) {
+  if (n <= 0) return 0;
+  int64_t grid = (n + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(taco_hip_fill_kernel<T>, dim3((unsigned)grid), dim3(256),
+                     0, taco_hip_stream(), p, n, v);
+  return (int)hipGetLastError();
+}
+
+/* ---- sorting ---- */
+
+__device__ inline void taco_hip_device_sort_i32(int32_t* a, int32_t n) {
+  for (int32_t i = 1; i < n; ++i) {
+    int32_t v = a[i], j = i - 1;
+    while (j >= 0 && a[j] > v) {
+      a[j + 1] = a[j];
+      --j;
+    }
+    a[j + 1] = v;
+  }
+}
+
+/* Ascending bitonic sort of one value per lane across the 64-lane wave. */
+template <typename K>
+__device__ inline K taco_hip_wave_sort(K v) {
+  const int lane = (int)__lane_id();
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const K p = __shfl_xor(v, j, 64);
+      const bool up = (lane & k) == 0;
+      const bool lower = (lane & j) == 0;
+      v = (lower == up) ? (p < v ? p : v) : (p > v ? p : v);
+    }
+  }
+  return v;
+}
+
+/* ---- SpGEMM rows, one 64-lane wave per row of C = A * B ----
+ * Pattern of row i: sorted distinct structural columns of the B rows named by
+ * A's row i.  Values: products summed per column in encounter order (jA
+ * outer, kB inner), the first stored, later ones added.
+ * Short rows (<= 64 candidates) live in lanes; long rows use a per-wave LDS
+ * hash set of at most TACO_HIP_ROW_CAP distinct columns.  Callers launch
+ * 256-thread workgroups (4 waves). */
+
+#define TACO_HIP_ROW_CAP 1024
+#define TACO_HIP_ROW_WAVES 4
+
+struct TacoHipRowSet {
+  int32_t keys[TACO_HIP_ROW_CAP];
+  int32_t count;
+};
+
+__device__ inline int64_t taco_hip_row_candidates(const int32_t* apos,
+                                                  const int32_t* acrd,
+                                                  const int32_t* bpos,
+                                                  int32_t i) {
+  const int lane = (int)__lane_id();
+  int64_t n = 0;
+  for (int32_t p = apos[i] + lane; p < apos[i + 1]; p += 64) {
+    const int32_t j = acrd[p];
+    n += bpos[j + 1] - bpos[j];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+  return n;
+}
+
+/* Candidate q of row i in encounter order (short rows). */
+__device__ inline bool taco_hip_row_candidate(
+    const int32_t* apos, const int32_t* acrd, const int32_t* bpos,
+    const int32_t* bcrd, int32_t i, int q, int32_t* col, int32_t* jA,
+    int32_t* kB) {
+  int acc = 0;
+  for (int32_t p = apos[i]; p < apos[i + 1]; ++p) {
+    const int32_t j = acrd[p];
+    const int32_t b0 = bpos[j], len = bpos[j + 1] - b0;
+    if (q < acc + len) {
+      *jA = p;
+      *kB = b0 + (q - acc);
+      *col = bcrd[b0 + (q - acc)];
+      return true;
+    }
+    acc += len;
+  }
+  return false;
+}
+
+__device__ inline void taco_hip_wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+/* Distinct columns of a long row into w (hash set).  -1 on overflow. */
+__device__ inline int32_t taco_hip_row_hash(TacoHipRowSet* w,
+                                            const int32_t* apos,
+                                            const int32_t* acrd,
+                                            const int32_t* bpos,
+                                            const int32_t* bcrd, int32_t i) {
+  const int lane = (int)__lane_id();
+  for (int q = lane; q < TACO_HIP_ROW_CAP; q += 64) w->keys[q] = -1;
+  if (lane == 0) w->count = 0;
+  taco_hip_wave_lds_sync();
+  bool overflow = false;
+  for (int32_t p = apos[i]; p < apos[i + 1]; ++p) {
+    const int32_t j = acrd[p];
+    for (int32_t kB = bpos[j] + lane; kB < bpos[j + 1]; kB += 64) {
+      const int32_t col = bcrd[kB];
+      uint32_t h = ((uint32_t)col * 2654435761u) & (TACO_HIP_ROW_CAP - 1);
+      for (int probe = 0; probe < TACO_HIP_ROW_CAP; ++probe) {
+        const int32_t prev = atomicCAS(&w->keys[h], -1, col);
+        if (prev == -1) {
+          atomicAdd(&w->count, 1);
+          break;
+        }
+        if (prev == col) break;
+        h = (h + 1) & (TACO_HIP_ROW_CAP - 1);
+        if (probe == TACO_HIP_ROW_CAP - 1) overflow = true;
+      }
+    }
+  }
+  taco_hip_wave_lds_sync();
+  const bool any_overflow = __ballot(overflow) != 0;
+  const int32_t cnt = w->count;
+  return any_overflow || cnt > TACO_HIP_ROW_CAP ? -1 : cnt;
+}
+
+/* Sorts the hash set's keys ascending into w->keys[0..count). */
+__device__ inline void taco_hip_row_hash_sort(TacoHipRowSet* w) {
+  const int lane = (int)__lane_id();
+  for (int q = lane; q < TACO_HIP_ROW_CAP; q += 64)
+    if (w->keys[q] == -1) w->keys[q] = INT_MAX;
+  taco_hip_wave_lds_sync();
+  for (int k = 2; k <= TACO_HIP_ROW_CAP; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int q = lane; q < TACO_HIP_ROW_CAP; q += 64) {
+        const int partner = q ^ j;
+        if (partner > q) {
+          const int32_t a = w->keys[q], b = w->keys[partner];
+          const bool up = (q & k) == 0;
+          if ((a > b) == up) {
+            w->keys[q] = b;
+            w->keys[partner] = a;
+          }
+        }
+      }
+      taco_hip_wave_lds_sync();
+    }
+  }
+}
+
+/* Symbolic row: returns the distinct count (count pass) or writes the sorted
+ * columns at ccrd[cpos_i ...] (fill pass).  status bit 1 on overflow. */
+template <bool FILL>
+__device__ inline int32_t taco_hip_spgemm_row_symbolic(
+    const int32_t* apos, const int32_t* acrd, const int32_t* bpos,
+    const int32_t* bcrd, int32_t i, int32_t cpos_i, int32_t* ccrd,
+    int32_t* status) {
+  __shared__ TacoHipRowSet s_sets[TACO_HIP_ROW_WAVES];
+  const int lane = (int)__lane_id();
+  const int64_t n = taco_hip_row_candidates(apos, acrd, bpos, i);
+  if (n <= 64) {
+    int32_t col = INT_MAX, jA = 0, kB = 0;
+    if (lane < n) taco_hip_row_candidate(apos, acrd, bpos, bcrd, i, lane, &col, &jA, &kB);
+    const int32_t s = taco_hip_wave_sort(col);
+    const int32_t prev = __shfl_up(s, 1, 64);
+    const bool head = s != INT_MAX && (lane == 0 || prev != s);
+    const uint64_t heads = __ballot(head);
+    if (FILL && head) {
+      const int rank = __popcll(heads & ((1ull << lane) - 1));
+      ccrd[cpos_i + rank] = s;
+    }
+    return __popcll(heads);
+  }
+  TacoHipRowSet* w = &s_sets[(threadIdx.x / 64) % TACO_HIP_ROW_WAVES];
+  const int32_t cnt = taco_hip_row_hash(w, apos, acrd, bpos, bcrd, i);
+  if (cnt < 0) {
+    if (lane == 0) atomicOr(status, 1);
+    return 0;
+  }
+  if (FILL) {
+    taco_hip_row_hash_sort(w);
+    for (int q = lane; q < cnt; q += 64) ccrd[cpos_i + q] = w->keys[q];
+  }
+  return cnt;
+}
+
+/* Numeric row on an assembled (sorted) pattern. */
+__device__ inline void taco_hip_spgemm_row_numeric(
+    const int32_t* apos, const int32_t* acrd, const double* avals,
+    const int32_t* bpos, const int32_t* bcrd, const double* bvals, int32_t i,
+    const int32_t* cpos, const int32_t* ccrd, double* cvals) {
+  __shared__ double s_acc[TACO_HIP_ROW_WAVES][TACO_HIP_ROW_CAP];
+  const int lane = (int)__lane_id();
+  const int64_t n = taco_hip_row_candidates(apos, acrd, bpos, i);
+  const int32_t c0 = cpos[i], nc = cpos[i + 1] - c0;
+  if (n <= 64) {
+    int32_t col = INT_MAX, jA = 0, kB = 0;
+    double prod = 0;
+    if (lane < n) {
+      taco_hip_row_candidate(apos, acrd, bpos, bcrd, i, lane, &col, &jA, &kB);
+      prod = avals[jA] * bvals[kB];
+    }
+    /* Key = column, then encounter index (which also names the lane). */
+    uint64_t key = ((uint64_t)(uint32_t)col << 32) | (uint32_t)lane;
+    key = taco_hip_wave_sort(key);
+    const int32_t scol = (int32_t)(key >> 32);
+    const double v = __shfl(prod, (int)(key & 63), 64);
+    const int32_t prev = __shfl_up(scol, 1, 64);
+    const bool valid = scol != INT_MAX;
+    const bool head = valid && (lane == 0 || prev != scol);
+    const uint64_t heads = __ballot(head);
+    const int nvalid = __popcll(__ballot(valid));
+    int run = 0;
+    if (head) {
+      const uint64_t above = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1);
+      const int next = above ? __ffsll((unsigned long long)above) - 1 : nvalid;
+      run = next - lane;
+    }
+    int maxrun = run;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const int o = __shfl_xor(maxrun, off, 64);
+      maxrun = o > maxrun ? o : maxrun;
+    }
+    double s = v;
+    for (int q = 1; q < maxrun; ++q) {
+      const double u = __shfl(v, (lane + q) & 63, 64);
+      if (head && q < run) s = s + u;
+    }
+    if (head) {
+      const int rank = __popcll(heads & ((1ull << lane) - 1));
+      if (rank < nc) cvals[c0 + rank] = s;
+    }
+    return;
+  }
+  if (nc > TACO_HIP_ROW_CAP) return; /* pattern not from the symbolic pass */
+  double* acc = s_acc[(threadIdx.x / 64) % TACO_HIP_ROW_WAVES];
+  for (int q = lane; q < nc; q += 64) acc[q] = 0;
+  taco_hip_wave_lds_sync();
+  const int32_t* crow = ccrd + c0;
+  for (int32_t p = apos[i]; p < apos[i + 1]; ++p) {
+    const int32_t j = acrd[p];
+    const double a = avals[p];
+    const int32_t bend = bpos[j + 1];
+    for (int32_t kb0 = bpos[j]; kb0 < bend; kb0 += 64) {
+      const int32_t kB = kb0 + lane;
+      int slot = -1;
+      double prod = 0;
+      if (kB < bend) {
+        const int32_t col = bcrd[kB];
+        prod = a * bvals[kB];
+        int lo = 0, hi = nc;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (crow[mid] < col) lo = mid + 1; else hi = mid;
+        }
+        slot = lo < nc && crow[lo] == col ? lo : -1;
+      }
+      const int lim = bend - kb0 < 64 ? bend - kb0 : 64;
+      for (int q = 0; q < lim; ++q) {
+        const int sq = __shfl(slot, q, 64);
+        const double vq = __shfl(prod, q, 64);
+        if (lane == 0 && sq >= 0) acc[sq] = acc[sq] + vq;
+        taco_hip_wave_lds_sync();
+      }
+    }
+  }
+  for (int q = lane; q < nc; q += 64) cvals[c0 + q] = acc[q];
+}
+
+#endif

@@ -84,7 +84,78 @@ _SIGS = {
     "init_taco_tensor_t": (P_T, [c_int32, c_int32, POINTER(c_int32),
                                  POINTER(c_int32), POINTER(c_int), c_void_p]),
     "deinit_taco_tensor_t": (None, [P_T]),
+    "taco_hip_device_exclusive_scan": (c_int, [c_void_p, c_int32, c_void_p]),
+    "taco_hip_workspace_get": (c_void_p, [c_size_t, c_int]),
+    "taco_hip_generate": (c_void_p, [c_void_p, c_int32]),
+    "taco_hip_free_string": (None, [c_void_p]),
+    "taco_hip_module_create": (c_void_p, []),
+    "taco_hip_module_set_source": (c_int, [c_void_p, c_char_p]),
+    "taco_hip_module_get_source": (c_char_p, [c_void_p]),
+    "taco_hip_module_compile": (c_int, [c_void_p]),
+    "taco_hip_module_path": (c_char_p, [c_void_p]),
+    "taco_hip_module_get_func": (c_void_p, [c_void_p, c_char_p]),
+    "taco_hip_module_call_packed": (c_int, [c_void_p, c_char_p, c_void_p]),
+    "taco_hip_module_destroy": (None, [c_void_p]),
+    "taco_hip_module_report": (c_int, [c_int, c_char_p]),
+    "taco_hip_module_finish": (c_int, [c_int]),
+    "taco_hip_host_read_bytes": (c_int, [c_void_p, c_void_p, c_size_t]),
 }
+
+
+class GenSpec(Structure):
+    """Mirror of ``taco_hip_gen_spec``."""
+    _fields_ = [
+        ("op", c_int32),
+        ("family", c_int32),
+        ("csize", c_int32),
+        ("block_size", c_int32),
+        ("nnz_per_thread", c_int32),
+        ("warp_size", c_int32),
+        ("rows_per_warp", c_int32),
+        ("bound", c_int32),
+    ]
+
+
+OP_SPMV, OP_SPMM, OP_SPGEMM = range(3)
+GEN_DEFAULT, GEN_WARP_ROWS, GEN_NNZ_SPLIT, GEN_TWO_PHASE = range(4)
+
+
+def generate(spec, what=0):
+    """Lowered IR (what=1) or HIP source (what=0) of a GenSpec."""
+    L = lib()
+    p = L.taco_hip_generate(ctypes.byref(spec), what)
+    if not p:
+        raise TacoHipError("generate failed: " +
+                           L.taco_hip_last_error().decode(errors="replace"))
+    try:
+        return ctypes.string_at(p).decode()
+    finally:
+        L.taco_hip_free_string(p)
+
+
+class Module:
+    """JIT module: source -> hipcc -> .so (content-hash cached) -> dlopen."""
+
+    def __init__(self, source):
+        L = lib()
+        self._m = L.taco_hip_module_create()
+        check(L.taco_hip_module_set_source(self._m, source.encode()), "set_source")
+
+    def compile(self):
+        check(lib().taco_hip_module_compile(self._m), "module compile")
+        return lib().taco_hip_module_path(self._m).decode()
+
+    def call(self, name, descs):
+        args = (c_void_p * len(descs))(*[ctypes.addressof(d.t) for d in descs])
+        return lib().taco_hip_module_call_packed(self._m, name.encode(), args)
+
+    def has(self, name):
+        return bool(lib().taco_hip_module_get_func(self._m, name.encode()))
+
+    def __del__(self):
+        if getattr(self, "_m", None) and _lib is not None:
+            _lib.taco_hip_module_destroy(self._m)
+            self._m = None
 
 EXPORTED = tuple(_SIGS)
 

@@ -31,7 +31,7 @@ struct Workspace {
   void* ptr = nullptr;
   size_t bytes = 0;
 };
-constexpr int kSlots = 4;
+constexpr int kSlots = 8;
 constexpr int kMaxDevices = 16;
 std::mutex g_ws_mutex;
 Workspace g_ws[kMaxDevices][kSlots];
@@ -313,6 +313,33 @@ void deinit_taco_tensor_t(taco_tensor_t* t) {
   free(t->mode_types);
   free(t->fill_value);
   free(t);
+}
+
+}  // extern "C"
+
+// ---- support for generated modules ----
+
+extern "C" {
+
+int taco_hip_module_report(int code, const char* what) {
+  taco_hip::fail("generated module: %s (code %d)", what ? what : "", code);
+  return code != 0 ? code : 1;
+}
+
+int taco_hip_module_finish(int rc) {
+  if (rc != 0) return rc;
+  return taco_hip::finish("generated module");
+}
+
+int taco_hip_host_read_bytes(void* dst, const void* src, size_t bytes) {
+  hipStream_t st = taco_hip::current_stream();
+  TH_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+  TH_CHECK(hipStreamSynchronize(st));
+  return 0;
+}
+
+void* taco_hip_workspace_get(size_t bytes, int slot) {
+  return taco_hip::workspace(bytes, slot);
 }
 
 }  // extern "C"

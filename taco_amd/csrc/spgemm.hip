@@ -12,192 +12,42 @@
 //             encounter order (jA outer, kB inner), first product stored,
 //             later ones added — the order of taco's workspace loop.
 //
-// Row kernels, one 64-lane wave per row:
+// Row kernels, one 64-lane wave per row, built on the row helpers of
+// include/taco_hip_rt.h (shared with modules emitted by CodeGen_HIP):
 //   short rows (<= 64 candidates): candidates in lanes, bitonic sort across
 //     the wave with shuffles.  The numeric phase sorts 64-bit keys
 //     (column << 32 | encounter index), so equal columns come out in
 //     encounter order and are summed sequentially in that order.
-//   long rows: a per-wave LDS hash set of up to kHashCap distinct columns,
-//     then an in-LDS bitonic sort of the distinct columns; the numeric phase
-//     accumulates into LDS slots, adding candidates one at a time in
-//     encounter order.
+//   long rows: a per-wave LDS hash set of up to TACO_HIP_ROW_CAP distinct
+//     columns, then an in-LDS bitonic sort; the numeric phase accumulates
+//     into LDS slots, adding candidates one at a time in encounter order.
 
 #include <climits>
 
 #include "common.h"
 #include "device.h"
+#include "taco_hip_rt.h"
 
 namespace taco_hip {
 namespace {
 
-constexpr int kWavesPerBlock = 4;
-constexpr int kHashCap = 1024;  // distinct columns per long row
+constexpr int kWavesPerBlock = TACO_HIP_ROW_WAVES;
+constexpr int kHashCap = TACO_HIP_ROW_CAP;
 
-template <typename K>
-__device__ __forceinline__ K wave_bitonic_sort(K v) {
-  const int lane = (int)__lane_id();
-#pragma unroll
-  for (int k = 2; k <= kWave; k <<= 1) {
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      const K p = __shfl_xor(v, j, kWave);
-      const bool up = (lane & k) == 0;
-      const bool lower = (lane & j) == 0;
-      v = (lower == up) ? (p < v ? p : v) : (p > v ? p : v);
-    }
-  }
-  return v;
-}
-
-// Candidate count of row i: sum of B row lengths over row i of A.
-__device__ __forceinline__ int64_t row_candidates(
-    const int32_t* apos, const int32_t* acrd, const int32_t* bpos, int32_t i) {
-  const int lane = (int)__lane_id();
-  int64_t n = 0;
-  for (int32_t p = apos[i] + lane; p < apos[i + 1]; p += kWave) {
-    const int32_t j = acrd[p];
-    n += bpos[j + 1] - bpos[j];
-  }
-  return wave_sum(n);
-}
-
-// Candidate q (0-based, encounter order) of row i for short rows: lane q
-// walks the row of A to find which B row holds its candidate.  Every lane
-// scans A's row; rows are short on this path (<= 64 candidates).
-__device__ __forceinline__ bool short_candidate(
-    const int32_t* apos, const int32_t* acrd, const int32_t* bpos,
-    const int32_t* bcrd, int32_t i, int q, int32_t* col, int32_t* jA_out,
-    int32_t* kB_out) {
-  int acc = 0;
-  for (int32_t p = apos[i]; p < apos[i + 1]; ++p) {
-    const int32_t j = acrd[p];
-    const int32_t b0 = bpos[j], len = bpos[j + 1] - b0;
-    if (q < acc + len) {
-      *jA_out = p;
-      *kB_out = b0 + (q - acc);
-      *col = bcrd[b0 + (q - acc)];
-      return true;
-    }
-    acc += len;
-  }
-  return false;
-}
-
-struct WaveLds {
-  int32_t keys[kHashCap];
-  int32_t count;
-};
-
-// Inserts every candidate column of row i into the wave's LDS hash set.
-// Returns the number of distinct columns, or -1 when it exceeds kHashCap.
-__device__ int32_t hash_row(WaveLds* w, const int32_t* apos,
-                            const int32_t* acrd, const int32_t* bpos,
-                            const int32_t* bcrd, int32_t i) {
-  const int lane = (int)__lane_id();
-  for (int q = lane; q < kHashCap; q += kWave) w->keys[q] = -1;
-  if (lane == 0) w->count = 0;
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  bool overflow = false;
-  for (int32_t p = apos[i]; p < apos[i + 1]; ++p) {
-    const int32_t j = acrd[p];
-    for (int32_t kB = bpos[j] + lane; kB < bpos[j + 1]; kB += kWave) {
-      const int32_t col = bcrd[kB];
-      uint32_t h = ((uint32_t)col * 2654435761u) & (kHashCap - 1);
-      for (int probe = 0; probe < kHashCap; ++probe) {
-        const int32_t prev = atomicCAS(&w->keys[h], -1, col);
-        if (prev == -1) {
-          atomicAdd(&w->count, 1);
-          break;
-        }
-        if (prev == col) break;
-        h = (h + 1) & (kHashCap - 1);
-        if (probe == kHashCap - 1) overflow = true;
-      }
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  const bool any_overflow = __ballot(overflow) != 0;
-  const int32_t cnt = w->count;
-  return any_overflow || cnt > kHashCap ? -1 : cnt;
-}
-
-// Sorts the distinct keys of the hash set into w->keys[0..cnt) ascending.
-__device__ void compact_sort(WaveLds* w, int32_t cnt) {
-  const int lane = (int)__lane_id();
-  // Compact: empty slots become INT_MAX and sort to the end.
-  for (int q = lane; q < kHashCap; q += kWave)
-    if (w->keys[q] == -1) w->keys[q] = INT_MAX;
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  for (int k = 2; k <= kHashCap; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int q = lane; q < kHashCap; q += kWave) {
-        const int partner = q ^ j;
-        if (partner > q) {
-          const int32_t a = w->keys[q], b = w->keys[partner];
-          const bool up = (q & k) == 0;
-          if ((a > b) == up) {
-            w->keys[q] = b;
-            w->keys[partner] = a;
-          }
-        }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-    }
-  }
-  (void)cnt;
-}
-
-// Symbolic pass.  fill == false: counts[i] = distinct columns of row i.
-// fill == true: writes row i's sorted columns to ccrd[cpos[i] ...].
-// status[0] is set to 1 when a long row exceeds kHashCap distinct columns.
+// Symbolic pass.  FILL == false: counts[i] = distinct columns of row i.
+// FILL == true: writes row i's sorted columns to ccrd[cpos[i] ...].
+// status bit 1 is set when a long row exceeds kHashCap distinct columns.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_spgemm_symbolic(
     int32_t m, const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
     const int32_t* __restrict__ bpos, const int32_t* __restrict__ bcrd,
     int32_t* __restrict__ counts, const int32_t* __restrict__ cpos,
     int32_t* __restrict__ ccrd, int32_t* __restrict__ status) {
-  __shared__ WaveLds s_w[kWavesPerBlock];
-  const int wave = threadIdx.x / kWave;
-  const int lane = threadIdx.x % kWave;
-  const int64_t i64 = (int64_t)blockIdx.x * kWavesPerBlock + wave;
-  if (i64 >= m) return;
-  const int32_t i = (int32_t)i64;
-  const int64_t n = row_candidates(apos, acrd, bpos, i);
-  if (n <= kWave) {
-    int32_t col = INT_MAX, jA = 0, kB = 0;
-    if (lane < n) short_candidate(apos, acrd, bpos, bcrd, i, lane, &col, &jA, &kB);
-    const int32_t s = wave_bitonic_sort(col);
-    const int32_t prev = __shfl_up(s, 1, kWave);
-    const bool head = s != INT_MAX && (lane == 0 || prev != s);
-    const uint64_t heads = __ballot(head);
-    if (!FILL) {
-      if (lane == 0) counts[i] = __popcll(heads);
-    } else if (head) {
-      const int rank = __popcll(heads & ((1ull << lane) - 1));
-      ccrd[cpos[i] + rank] = s;
-    }
-    return;
-  }
-  WaveLds* w = &s_w[wave];
-  const int32_t cnt = hash_row(w, apos, acrd, bpos, bcrd, i);
-  if (cnt < 0) {
-    if (lane == 0) {
-      atomicOr(status, 1);
-      if (!FILL) counts[i] = 0;
-    }
-    return;
-  }
-  if (!FILL) {
-    if (lane == 0) counts[i] = cnt;
-    return;
-  }
-  compact_sort(w, cnt);
-  const int32_t base = cpos[i];
-  for (int q = lane; q < cnt; q += kWave) ccrd[base + q] = w->keys[q];
+  const int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (i >= m) return;
+  const int32_t c = taco_hip_spgemm_row_symbolic<FILL>(
+      apos, acrd, bpos, bcrd, (int32_t)i, FILL ? cpos[i] : 0, ccrd, status);
+  if (!FILL && __lane_id() == 0) counts[i] = c;
 }
 
 // Numeric pass on the assembled pattern.
@@ -207,95 +57,10 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(
     const int32_t* __restrict__ bcrd, const double* __restrict__ bvals,
     const int32_t* __restrict__ cpos, const int32_t* __restrict__ ccrd,
     double* __restrict__ cvals) {
-  __shared__ double s_acc[kWavesPerBlock][kHashCap];
-  const int wave = threadIdx.x / kWave;
-  const int lane = threadIdx.x % kWave;
-  const int64_t i64 = (int64_t)blockIdx.x * kWavesPerBlock + wave;
-  if (i64 >= m) return;
-  const int32_t i = (int32_t)i64;
-  const int64_t n = row_candidates(apos, acrd, bpos, i);
-  const int32_t c0 = cpos[i], nc = cpos[i + 1] - c0;
-  if (n <= kWave) {
-    int32_t col = INT_MAX, jA = 0, kB = 0;
-    double prod = 0;
-    if (lane < n) {
-      short_candidate(apos, acrd, bpos, bcrd, i, lane, &col, &jA, &kB);
-      prod = avals[jA] * bvals[kB];
-    }
-    // Key: column, then encounter index; carries the product's source lane.
-    uint64_t key = ((uint64_t)(uint32_t)col << 32) | (uint32_t)lane;
-    key = wave_bitonic_sort(key);
-    const int32_t scol = (int32_t)(key >> 32);
-    const int src = (int)(key & 63);
-    const double v = __shfl(prod, src, kWave);
-    const int32_t prev = __shfl_up(scol, 1, kWave);
-    const bool valid = scol != INT_MAX;
-    const bool head = valid && (lane == 0 || prev != scol);
-    const uint64_t heads = __ballot(head);
-    const int nvalid = __popcll(__ballot(valid));
-    // Run of a head: lanes up to the next head (or the end of valid lanes).
-    int run = 0;
-    if (head) {
-      const uint64_t above = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1);
-      const int next = above ? __ffsll((unsigned long long)above) - 1 : nvalid;
-      run = next - lane;
-    }
-    int maxrun = run;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const int o = __shfl_xor(maxrun, off, kWave);
-      maxrun = o > maxrun ? o : maxrun;
-    }
-    // Sequential sum of each run in encounter order: s = v0; s = s + v1; ...
-    double s = v;
-    for (int q = 1; q < maxrun; ++q) {
-      const double u = __shfl(v, (lane + q) & 63, kWave);
-      if (head && q < run) s = s + u;
-    }
-    if (head) {
-      const int rank = __popcll(heads & ((1ull << lane) - 1));
-      if (rank < nc) cvals[c0 + rank] = s;
-    }
-    return;
-  }
-  // Long row: LDS slot per pattern entry (nc <= kHashCap was checked by the
-  // symbolic pass), candidates added one at a time in encounter order.
-  // A caller-supplied pattern longer than the slot array is left unwritten
-  // rather than overrunning LDS.
-  if (nc > kHashCap) return;
-  double* acc = s_acc[wave];
-  for (int q = lane; q < nc; q += kWave) acc[q] = 0;
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  const int32_t* crow = ccrd + c0;
-  for (int32_t p = apos[i]; p < apos[i + 1]; ++p) {
-    const int32_t j = acrd[p];
-    const double a = avals[p];
-    for (int32_t kb0 = bpos[j]; kb0 < bpos[j + 1]; kb0 += kWave) {
-      const int32_t kB = kb0 + lane;
-      int slot = -1;
-      double prod = 0;
-      if (kB < bpos[j + 1]) {
-        const int32_t col = bcrd[kB];
-        prod = a * bvals[kB];
-        int lo = 0, hi = nc;  // crow sorted ascending
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (crow[mid] < col) lo = mid + 1; else hi = mid;
-        }
-        slot = lo < nc && crow[lo] == col ? lo : -1;
-      }
-      const int lim = bpos[j + 1] - kb0 < kWave ? bpos[j + 1] - kb0 : kWave;
-      for (int q = 0; q < lim; ++q) {
-        const int sq = __shfl(slot, q, kWave);
-        const double vq = __shfl(prod, q, kWave);
-        if (lane == 0 && sq >= 0) acc[sq] = acc[sq] + vq;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
-  }
-  for (int q = lane; q < nc; q += kWave) cvals[c0 + q] = acc[q];
+  const int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (i >= m) return;
+  taco_hip_spgemm_row_numeric(apos, acrd, avals, bpos, bcrd, bvals, (int32_t)i,
+                              cpos, ccrd, cvals);
 }
 
 // ---- counts -> pos (exclusive prefix sum with pos[m] = total) ----
@@ -500,4 +265,18 @@ extern "C" int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,
       a.m, a.pos, a.crd, (const double*)a.vals, b.pos, b.crd,
       (const double*)b.vals, c.pos, c.crd, (double*)c.vals);
   return finish("taco_hip_spgemm_compute");
+}
+
+extern "C" int taco_hip_device_exclusive_scan(const int32_t* counts, int32_t m,
+                                              int32_t* pos) {
+  hipStream_t st = current_stream();
+  int32_t* status = (int32_t*)workspace(sizeof(int32_t), 7);
+  if (!status) return 1;
+  TH_CHECK(hipMemsetAsync(status, 0, sizeof(int32_t), st));
+  if (int rc = device_counts_to_pos(m, counts, pos, status, st)) return rc;
+  int32_t host = 0;
+  TH_CHECK(hipMemcpyAsync(&host, status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  TH_CHECK(hipStreamSynchronize(st));
+  TH_REQUIRE(host == 0, "exclusive scan: total exceeds the int32 index range");
+  return 0;
 }

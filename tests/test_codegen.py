@@ -1,0 +1,92 @@
+"""CodeGen_HIP and the JIT module loader, on the CPU (no GPU calls): the
+lowered IR carries the schedule's parallel units, the emitted module has the
+taco module ABI, modules compile through the cache, and failures are
+reported, not fatal."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+from taco_amd import _lib, codegen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.parametrize("name", sorted(codegen.STANDARD_SPECS))
+def test_module_abi(name):
+    src = _lib.generate(codegen.standard(name))
+    for fn in ("assemble", "compute", "evaluate"):
+        assert f'extern "C" int {fn}(taco_tensor_t*' in src
+        assert f'extern "C" int _shim_{fn}(void** parameterPack)' in src
+    assert "__global__ __launch_bounds__(" in src
+    assert "hipLaunchKernelGGL(computeDeviceKernel0" in src
+    # no device-wide sync after launches, no CUDA spellings
+    for bad in ("cudaDeviceSynchronize", "hipDeviceSynchronize", "__syncwarp",
+                "__shfl_sync", "cudaMallocManaged", "hipMemsetAsync"):
+        assert bad not in src, bad
+
+
+def test_ir_default_schedule_units():
+    ir = _lib.generate(codegen.standard("spmv_default_f64"), 1)
+    assert "for i0 in [0, ((A1_dimension + 255) / 256)) step 1 GPUBlock" in ir
+    assert "for i1 in [0, 256) step 1 GPUThread" in ir
+
+
+def test_ir_warp_rows_units():
+    ir = _lib.generate(codegen.standard("spmv_rows_f64"), 1)
+    for unit in ("GPUBlock", "GPUWarp", "GPUThread", "GPUWarpReduction"):
+        assert unit in ir
+    src = _lib.generate(codegen.standard("spmv_rows_f64"))
+    assert "taco_hip_group_sum<32>" in src
+    assert "threadIdx.x / 32" in src and "threadIdx.x % 32" in src
+
+
+def test_ir_nnz_split_atomics():
+    ir = _lib.generate(codegen.standard("spmv_nnz_f64"), 1)
+    assert "[atomic GPUThread]" in ir and "[atomic GPUWarp]" in ir
+    src = _lib.generate(codegen.standard("spmv_nnz_f64"))
+    assert "taco_hip_binarySearchBeforeBlockLaunch" in src
+    assert "taco_hip_atomicAddWave(y_vals" in src
+    assert "#pragma unroll 8" in src
+
+
+def test_kernel_parameters_are_values_not_references():
+    src = _lib.generate(codegen.standard("spmm_nnz_f64"))
+    kernel_sig = [l for l in src.splitlines() if l.startswith("void computeDeviceKernel0(")][0]
+    assert "&" not in kernel_sig
+    assert "int32_t A_nnz" in kernel_sig
+
+
+def test_unsupported_spec_reports_error():
+    bad = codegen.spec(_lib.OP_SPGEMM, _lib.GEN_NNZ_SPLIT)
+    with pytest.raises(_lib.TacoHipError, match="two-phase"):
+        _lib.generate(bad)
+    with pytest.raises(_lib.TacoHipError, match="f64 and f32"):
+        _lib.generate(codegen.spec(_lib.OP_SPMV, 0, csize=16))
+
+
+def test_module_compiles_and_caches():
+    k1 = codegen.Kernel(codegen.standard("spmv_default_f64"))
+    assert os.path.exists(k1.path) and k1.module.has("_shim_compute")
+    k2 = codegen.Kernel(codegen.standard("spmv_default_f64"))
+    assert k2.path == k1.path
+
+
+def test_compile_failure_is_reported(tmp_path):
+    m = _lib.Module("this is not HIP code;\n")
+    with pytest.raises(_lib.TacoHipError, match="failed"):
+        m.compile()
+
+
+def test_missing_compiler_is_reported(tmp_path):
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from taco_amd import _lib\n"
+        "m = _lib.Module('// fresh module %s\\n')\n"
+        "try:\n    m.compile()\nexcept _lib.TacoHipError as e:\n    print('ERR', e)\n"
+    ) % (ROOT, os.getpid())
+    env = dict(os.environ, TACO_HIPCC="/nonexistent/hipcc", TACO_HIP_CACHE=str(tmp_path))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                         text=True, timeout=120)
+    assert "ERR" in out.stdout and "/nonexistent/hipcc" in out.stdout
