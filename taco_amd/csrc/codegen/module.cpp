@@ -146,9 +146,13 @@ int taco_hip_module_compile(taco_hip_module* m) {
   const std::string flags =
       env_or("TACO_HIPCCFLAGS", taco_hip_get_default_compiler_flags());
   const std::string cache = env_or("TACO_HIP_CACHE", libdir + "/_jit_cache");
+  // The key covers everything the object depends on: the source, the
+  // compiler and flags, and the runtime header the source includes.
+  const std::string rt = read_file(root + "/include/taco_hip_rt.h") +
+                         read_file(root + "/include/taco/taco_tensor_t.h");
   char key[32];
   snprintf(key, sizeof key, "%016llx",
-           (unsigned long long)fnv1a(m->source, fnv1a(cc + "\n" + flags)));
+           (unsigned long long)fnv1a(m->source, fnv1a(cc + "\n" + flags + "\n" + rt)));
   const std::string so = cache + "/" + key + ".so";
   std::lock_guard<std::mutex> lock(g_compile_mutex);
   if (!file_exists(so)) {
