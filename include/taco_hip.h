@@ -156,6 +156,25 @@ int taco_hip_spgemm_assemble(taco_tensor_t* C, taco_tensor_t* A,
 int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,
                             taco_tensor_t* B);
 
+/* Slab-tiled SpMV plan (f64).  A storage conversion of a CSR matrix for
+ * SpMV whose x exceeds one XCD's L2: columns split into slabs of slab_width
+ * (<= 2^18; 0 = 2^18, 2 MB of x), rows into `parts` workgroup parts (0 = 512)
+ * whose entries are reordered slab by slab, so that resident workgroups sweep
+ * the slabs in step and gather x from L2.  No reference counterpart: the
+ * reference's nearest analogue is packing a tensor into another format
+ * before compute (src/tensor.cpp pack()).  Built on the device from A's
+ * arrays; A's values are copied, so a plan reflects A at creation time.
+ * compute: y = A x with the planned A; row sums are accumulated with LDS
+ * atomics, so the summation order (and last bits) may vary run to run. */
+typedef struct taco_hip_spmv_plan taco_hip_spmv_plan;
+int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
+                              int32_t slab_width, taco_hip_spmv_plan** out);
+int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* plan, taco_tensor_t* y,
+                               taco_tensor_t* x, int32_t unroll);
+void taco_hip_spmv_plan_destroy(taco_hip_spmv_plan* plan);
+/* info[0..4] = parts, slabs, slab width, row weight, max rows in a part. */
+int taco_hip_spmv_plan_info(const taco_hip_spmv_plan* plan, int64_t* info);
+
 /* Device exclusive scan: pos[0] = 0, pos[i+1] = pos[i] + counts[i], on the
  * current stream.  Returns non-zero (and records an error) if the total
  * exceeds the int32 index range; synchronises the stream to check. */

@@ -84,6 +84,10 @@ _SIGS = {
     "init_taco_tensor_t": (P_T, [c_int32, c_int32, POINTER(c_int32),
                                  POINTER(c_int32), POINTER(c_int), c_void_p]),
     "deinit_taco_tensor_t": (None, [P_T]),
+    "taco_hip_spmv_plan_create": (c_int, [P_T, c_int32, c_int32, POINTER(c_void_p)]),
+    "taco_hip_spmv_plan_compute": (c_int, [c_void_p, P_T, P_T, c_int32]),
+    "taco_hip_spmv_plan_destroy": (None, [c_void_p]),
+    "taco_hip_spmv_plan_info": (c_int, [c_void_p, POINTER(c_int64)]),
     "taco_hip_device_exclusive_scan": (c_int, [c_void_p, c_int32, c_void_p]),
     "taco_hip_workspace_get": (c_void_p, [c_size_t, c_int]),
     "taco_hip_generate": (c_void_p, [c_void_p, c_int32]),
@@ -189,3 +193,29 @@ def check(rc, what="taco_hip call"):
         msg = lib().taco_hip_last_error().decode(errors="replace")
         raise TacoHipError(f"{what} failed ({rc}): {msg}")
     return rc
+
+
+class SpmvPlan:
+    """Slab-tiled SpMV plan over a CSR descriptor (see include/taco_hip.h)."""
+
+    def __init__(self, A_desc, parts=0, slab_width=0):
+        L = lib()
+        self._p = c_void_p()
+        check(L.taco_hip_spmv_plan_create(A_desc.ptr, parts, slab_width,
+                                          ctypes.byref(self._p)), "spmv plan")
+        self._keep = A_desc
+
+    def info(self):
+        out = (c_int64 * 5)()
+        check(lib().taco_hip_spmv_plan_info(self._p, out))
+        return dict(zip(("parts", "slabs", "slab_width", "row_weight",
+                         "max_local_rows"), list(out)))
+
+    def compute(self, y_desc, x_desc, unroll=0):
+        check(lib().taco_hip_spmv_plan_compute(self._p, y_desc.ptr, x_desc.ptr,
+                                               unroll), "spmv plan compute")
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.taco_hip_spmv_plan_destroy(self._p)
+            self._p = None

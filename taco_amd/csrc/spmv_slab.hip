@@ -1,0 +1,394 @@
+// Slab-tiled SpMV: an MI355X storage plan for CSR SpMV whose x does not fit
+// one XCD's L2.
+//
+// Random gathers x[crd] over a vector larger than the 4 MB L2 of an XCD are
+// served from the memory-side MALL at a fraction of the L2 hit rate, and they
+// dominate SpMV on matrices with scattered columns.  The plan splits the
+// columns into slabs of `slab_width` (2^18 columns = 2 MB of f64 by default)
+// and the rows into P parts sized so that one part's row partial sums fit in
+// LDS and all parts are resident at once.  Inside each part the entries are
+// reordered by slab (stable, so rows stay ordered within a slab).  Every
+// workgroup then sweeps slab 0, 1, 2, ... over its own entries; since all
+// workgroups progress at the same rate, each XCD's L2 holds the slab being
+// swept and the gathers hit L2.  A is still read exactly once.
+//
+// Partition: a weighted merge path over (row ends, entries) in which a row
+// end weighs `row_weight` entries; part p starts at (r_p, z_p) with
+// pos[r_p] <= z_p <= pos[r_p + 1], owns local rows r_p .. r_{p+1}, completes
+// rows r_p .. r_{p+1}-1 (writing y) and carries the partial sum of row
+// r_{p+1}; carries are added in part order by a fix-up kernel, as in the
+// merge-path kernel.
+//
+// Packed entry: (local row << 18) | (column - slab * slab_width), so an
+// entry stays 4 + 8 bytes.
+//
+// Row partial sums accumulate with LDS float atomics, so the order of the
+// additions inside a row is not fixed: results agree with the sequential sum
+// to rounding (the value tolerance) but may differ in the last bits from run
+// to run.  The MERGE_PATH schedule is the deterministic one.
+
+#include <climits>
+
+#include "common.h"
+#include "device.h"
+
+namespace taco_hip {
+
+namespace {
+
+constexpr int kColBits = 18;
+constexpr uint32_t kColMask = (1u << kColBits) - 1;
+constexpr int kPlanThreads = 512;
+constexpr int kMaxLocalRows = 9728;  // 76 KB of f64 partials per workgroup
+
+// Largest r in [0, m] with w*r + pos[r] <= d.
+__device__ int64_t weighted_row(const int32_t* pos, int64_t m, int64_t w,
+                                int64_t d) {
+  int64_t lo = 0, hi = m;  // f(lo) <= d holds for lo = 0 when d >= 0
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) / 2;
+    if (w * mid + (int64_t)pos[mid] <= d) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_slab_partition(
+    int32_t m, int64_t nnz, int64_t w, int64_t per_part, int32_t parts,
+    const int32_t* __restrict__ pos, int32_t* __restrict__ part_row,
+    int32_t* __restrict__ part_entry) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > parts) return;
+  const int64_t total = w * (int64_t)m + nnz;
+  int64_t d = p * per_part;
+  if (d > total || p == parts) d = total;
+  int64_t r = weighted_row(pos, m, w, d);
+  int64_t z = d - w * r;
+  if (r < m) {
+    if (z > pos[r + 1]) z = pos[r + 1];
+  } else {
+    r = m;
+    z = nnz;
+  }
+  if (z < pos[r]) z = pos[r];
+  part_row[p] = (int32_t)r;
+  part_entry[p] = (int32_t)z;
+}
+
+// Per part: entries per slab, then offsets off[p*(S+1) + s].
+__global__ __launch_bounds__(256) void k_slab_offsets(
+    int32_t parts, int32_t slabs, int32_t slab_width,
+    const int32_t* __restrict__ crd, const int32_t* __restrict__ part_entry,
+    int32_t* __restrict__ off) {
+  __shared__ int32_t s_count[1024];
+  const int32_t p = blockIdx.x;
+  for (int s = threadIdx.x; s < slabs; s += blockDim.x) s_count[s] = 0;
+  __syncthreads();
+  const int32_t z0 = part_entry[p], z1 = part_entry[p + 1];
+  for (int32_t z = z0 + threadIdx.x; z < z1; z += blockDim.x)
+    atomicAdd(&s_count[crd[z] / slab_width], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t acc = z0;
+    int32_t* o = off + (int64_t)p * (slabs + 1);
+    for (int s = 0; s < slabs; ++s) {
+      o[s] = acc;
+      acc += s_count[s];
+    }
+    o[slabs] = acc;
+  }
+}
+
+// Per part: stable scatter of the part's entries into slab order, packing
+// the local row into the column word.  Entries are taken in chunks of the
+// workgroup size; the rank of an entry inside its slab is the number of
+// earlier entries of the same slab in earlier chunks (running cursor) plus
+// in earlier lanes of this chunk (ballots per wave, prefix over waves).
+__global__ __launch_bounds__(256) void k_slab_scatter(
+    int32_t slabs, int32_t slab_width, const int32_t* __restrict__ pos,
+    const int32_t* __restrict__ crd, const double* __restrict__ vals,
+    const int32_t* __restrict__ part_row, const int32_t* __restrict__ part_entry,
+    const int32_t* __restrict__ off, uint32_t* __restrict__ ent,
+    double* __restrict__ val2) {
+  __shared__ int32_t s_cursor[1024];
+  __shared__ int32_t s_wave[4][1024];
+  const int32_t p = blockIdx.x;
+  const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const int32_t* o = off + (int64_t)p * (slabs + 1);
+  for (int s = tid; s < slabs; s += blockDim.x) s_cursor[s] = o[s];
+  const int32_t r0 = part_row[p], r1 = part_row[p + 1];
+  const int32_t z0 = part_entry[p], z1 = part_entry[p + 1];
+  __syncthreads();
+  for (int32_t base = z0; base < z1; base += 256) {
+    const int32_t z = base + tid;
+    const bool valid = z < z1;
+    int32_t s = -1, col = 0, lrow = 0;
+    double v = 0;
+    if (valid) {
+      col = crd[z];
+      v = vals[z];
+      s = col / slab_width;
+      // local row: the row of entry z within [r0, r1]
+      int32_t lo = r0, hi = r1;  // pos[lo] <= z, answer in [r0, r1]
+      while (lo < hi) {
+        const int32_t mid = lo + (hi - lo + 1) / 2;
+        if (pos[mid] <= z) lo = mid; else hi = mid - 1;
+      }
+      lrow = lo - r0;
+    }
+    // Rank within this chunk for each lane's slab.
+    for (int q = tid; q < 4 * slabs; q += blockDim.x) s_wave[q / slabs][q % slabs] = 0;
+    __syncthreads();
+    int rank_in_wave = 0;
+    {
+      // lanes with the same slab in this wave, below this lane
+      uint64_t below = (1ull << lane) - 1;
+      for (int ss = 0; ss < slabs; ++ss) {
+        const uint64_t mask = __ballot(s == ss);
+        if (s == ss) rank_in_wave = __popcll(mask & below);
+        if (lane == 0) s_wave[wave][ss] = __popcll(mask);
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      int32_t before = 0;
+      for (int w2 = 0; w2 < wave; ++w2) before += s_wave[w2][s];
+      const int32_t dst = s_cursor[s] + before + rank_in_wave;
+      ent[dst] = ((uint32_t)lrow << kColBits) | (uint32_t)(col - s * slab_width);
+      val2[dst] = v;
+    }
+    __syncthreads();
+    for (int ss = tid; ss < slabs; ss += blockDim.x)
+      s_cursor[ss] += s_wave[0][ss] + s_wave[1][ss] + s_wave[2][ss] + s_wave[3][ss];
+    __syncthreads();
+  }
+}
+
+template <int U>
+__global__ __launch_bounds__(kPlanThreads) void k_slab_spmv(
+    int32_t m, int32_t slabs, int32_t slab_width,
+    const int32_t* __restrict__ part_row, const int32_t* __restrict__ off,
+    const uint32_t* __restrict__ ent, const double* __restrict__ val2,
+    const double* __restrict__ x, double* __restrict__ y,
+    int32_t* __restrict__ carry_row, double* __restrict__ carry_val) {
+  __shared__ double s_part[kMaxLocalRows];
+  const int32_t p = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int32_t r0 = part_row[p], r1 = part_row[p + 1];
+  const int32_t nrows = r1 - r0 + 1;  // local rows, <= kMaxLocalRows by plan
+  for (int32_t q = tid; q < nrows; q += kPlanThreads) s_part[q] = 0;
+  __syncthreads();
+  const int32_t* o = off + (int64_t)p * (slabs + 1);
+  for (int32_t s = 0; s < slabs; ++s) {
+    const int32_t e0 = o[s], e1 = o[s + 1];
+    const double* xs = x + (int64_t)s * slab_width;
+    for (int32_t base = e0; base < e1; base += kPlanThreads * U) {
+      uint32_t pk[U];
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t e = base + tid + u * kPlanThreads;
+        pk[u] = e < e1 ? stream_load(ent + e) : 0u;
+        v[u] = e < e1 ? stream_load(val2 + e) : 0.0;
+      }
+      double xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t e = base + tid + u * kPlanThreads;
+        xv[u] = e < e1 ? xs[pk[u] & kColMask] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t e = base + tid + u * kPlanThreads;
+        if (e < e1) atomicAdd(&s_part[pk[u] >> kColBits], v[u] * xv[u]);
+      }
+    }
+  }
+  __syncthreads();
+  // Rows r0 .. r1-1 complete here; row r1 is carried.
+  for (int32_t q = tid; q < nrows - 1; q += kPlanThreads) y[r0 + q] = s_part[q];
+  if (tid == 0) {
+    carry_row[p] = r1;
+    carry_val[p] = s_part[nrows - 1];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_slab_fixup(
+    int32_t parts, int32_t m, const int32_t* __restrict__ carry_row,
+    const double* __restrict__ carry_val, double* __restrict__ y) {
+  const int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= parts) return;
+  const int32_t r = carry_row[b];
+  if (r >= m) return;
+  if (b > 0 && carry_row[b - 1] == r) return;
+  double s = carry_val[b];
+  for (int32_t c = b + 1; c < parts && carry_row[c] == r; ++c) s += carry_val[c];
+  y[r] += s;
+}
+
+__global__ void k_max_rows(int32_t parts, const int32_t* __restrict__ part_row,
+                           int32_t* __restrict__ out) {
+  const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= parts) return;
+  atomicMax(out, part_row[p + 1] - part_row[p] + 1);
+}
+
+}  // namespace
+}  // namespace taco_hip
+
+using namespace taco_hip;
+
+struct taco_hip_spmv_plan {
+  int32_t m = 0, n = 0;
+  int64_t nnz = 0;
+  int32_t parts = 0, slabs = 0, slab_width = 0;
+  int64_t row_weight = 0;
+  int32_t* part_row = nullptr;    // parts + 1
+  int32_t* part_entry = nullptr;  // parts + 1
+  int32_t* off = nullptr;         // parts * (slabs + 1)
+  uint32_t* ent = nullptr;        // nnz
+  double* val2 = nullptr;         // nnz
+  int32_t* carry_row = nullptr;   // parts
+  double* carry_val = nullptr;    // parts
+  int32_t max_local_rows = 0;
+};
+
+namespace {
+void plan_free(taco_hip_spmv_plan* p) {
+  if (!p) return;
+  for (void* q : {(void*)p->part_row, (void*)p->part_entry, (void*)p->off,
+                  (void*)p->ent, (void*)p->val2, (void*)p->carry_row,
+                  (void*)p->carry_val})
+    if (q) (void)hipFree(q);
+  delete p;
+}
+}  // namespace
+
+extern "C" int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
+                                         int32_t slab_width,
+                                         taco_hip_spmv_plan** out) {
+  TH_REQUIRE(out, "spmv plan: null output");
+  *out = nullptr;
+  CsrView a;
+  if (int rc = view_csr(A, "A", &a)) return rc;
+  TH_REQUIRE(a.vt == VType::F64, "spmv plan: f64 values only");
+  if (slab_width <= 0) slab_width = 1 << kColBits;
+  TH_REQUIRE(slab_width <= (1 << kColBits), "spmv plan: slab width above 2^18");
+  if (parts <= 0) parts = 512;
+  const int32_t slabs = a.n == 0 ? 1 : (int32_t)(((int64_t)a.n + slab_width - 1) / slab_width);
+  TH_REQUIRE(slabs <= 1024, "spmv plan: more than 1024 slabs");
+  // Row weight: parts * rows_per_part must cover the rows with a margin,
+  // rows_per_part = (w*m + nnz) / (parts * w) <= kMaxLocalRows - 2.
+  const int64_t cap = kMaxLocalRows - 2;
+  int64_t w = 1;
+  while (w < (1 << 20)) {
+    const int64_t per = (w * (int64_t)a.m + a.nnz + parts - 1) / parts;
+    if (per / w + 1 <= cap) break;
+    w *= 2;
+  }
+  const int64_t per_part = (w * (int64_t)a.m + a.nnz + parts - 1) / parts;
+  TH_REQUIRE(per_part / w + 1 <= cap,
+             "spmv plan: %d parts cannot hold %d rows; use more parts", parts, a.m);
+  TH_REQUIRE(per_part < INT32_MAX, "spmv plan: problem too large");
+
+  auto* p = new taco_hip_spmv_plan();
+  p->m = a.m;
+  p->n = a.n;
+  p->nnz = a.nnz;
+  p->parts = parts;
+  p->slabs = slabs;
+  p->slab_width = slab_width;
+  p->row_weight = w;
+  hipStream_t st = current_stream();
+  auto alloc = [&](void** q, size_t bytes) {
+    return hipMalloc(q, bytes < 16 ? 16 : bytes);
+  };
+  if (alloc((void**)&p->part_row, sizeof(int32_t) * (parts + 1)) ||
+      alloc((void**)&p->part_entry, sizeof(int32_t) * (parts + 1)) ||
+      alloc((void**)&p->off, sizeof(int32_t) * (size_t)parts * (slabs + 1)) ||
+      alloc((void**)&p->ent, sizeof(uint32_t) * (size_t)a.nnz) ||
+      alloc((void**)&p->val2, sizeof(double) * (size_t)a.nnz) ||
+      alloc((void**)&p->carry_row, sizeof(int32_t) * parts) ||
+      alloc((void**)&p->carry_val, sizeof(double) * parts)) {
+    plan_free(p);
+    return fail("spmv plan: device allocation failed");
+  }
+  k_slab_partition<<<(parts + 1 + 255) / 256, 256, 0, st>>>(
+      a.m, a.nnz, w, per_part, parts, a.pos, p->part_row, p->part_entry);
+  int32_t* maxrows = (int32_t*)workspace(sizeof(int32_t), 3);
+  if (!maxrows) {
+    plan_free(p);
+    return 1;
+  }
+  (void)hipMemsetAsync(maxrows, 0, sizeof(int32_t), st);
+  k_max_rows<<<(parts + 255) / 256, 256, 0, st>>>(parts, p->part_row, maxrows);
+  int32_t host_max = 0;
+  if (hipMemcpyAsync(&host_max, maxrows, sizeof(int32_t), hipMemcpyDeviceToHost, st) ||
+      hipStreamSynchronize(st)) {
+    plan_free(p);
+    return fail("spmv plan: partition failed");
+  }
+  if (host_max > kMaxLocalRows) {
+    plan_free(p);
+    return fail("spmv plan: a part holds %d rows (max %d)", host_max, kMaxLocalRows);
+  }
+  p->max_local_rows = host_max;
+  if (a.nnz > 0) {
+    k_slab_offsets<<<parts, 256, 0, st>>>(parts, slabs, slab_width, a.crd,
+                                         p->part_entry, p->off);
+    k_slab_scatter<<<parts, 256, 0, st>>>(slabs, slab_width, a.pos, a.crd,
+                                         (const double*)a.vals, p->part_row,
+                                         p->part_entry, p->off, p->ent, p->val2);
+  } else {
+    (void)hipMemsetAsync(p->off, 0, sizeof(int32_t) * (size_t)parts * (slabs + 1), st);
+  }
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    plan_free(p);
+    return fail("spmv plan: build kernels failed");
+  }
+  *out = p;
+  return 0;
+}
+
+extern "C" int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* p,
+                                          taco_tensor_t* y, taco_tensor_t* x,
+                                          int32_t unroll) {
+  TH_REQUIRE(p, "spmv plan: null plan");
+  DenseView xv, yv;
+  if (int rc = view_dense(x, "x", 1, &xv)) return rc;
+  if (int rc = view_dense(y, "y", 1, &yv)) return rc;
+  TH_REQUIRE(xv.vt == VType::F64 && yv.vt == VType::F64, "spmv plan: f64 x and y");
+  TH_REQUIRE(xv.dims[0] == p->n && yv.dims[0] == p->m,
+             "spmv plan: x/y sizes do not match the planned matrix");
+  if (p->m == 0) return 0;
+  hipStream_t st = current_stream();
+  const double* xp = (const double*)xv.vals;
+  double* yp = (double*)yv.vals;
+#define TH_SLAB(U)                                                          \
+  case U:                                                                   \
+    k_slab_spmv<U><<<p->parts, kPlanThreads, 0, st>>>(                      \
+        p->m, p->slabs, p->slab_width, p->part_row, p->off, p->ent,         \
+        p->val2, xp, yp, p->carry_row, p->carry_val);                       \
+    break;
+  switch (unroll <= 0 ? 4 : unroll) {
+    TH_SLAB(1) TH_SLAB(2) TH_SLAB(4) TH_SLAB(8)
+    default:
+      return fail("spmv plan: unroll %d unsupported", unroll);
+  }
+#undef TH_SLAB
+  TH_LAUNCH_CHECK("k_slab_spmv");
+  k_slab_fixup<<<(p->parts + 255) / 256, 256, 0, st>>>(p->parts, p->m, p->carry_row,
+                                                       p->carry_val, yp);
+  return finish("taco_hip_spmv_plan_compute");
+}
+
+extern "C" void taco_hip_spmv_plan_destroy(taco_hip_spmv_plan* p) { plan_free(p); }
+
+extern "C" int taco_hip_spmv_plan_info(const taco_hip_spmv_plan* p, int64_t* info) {
+  TH_REQUIRE(p && info, "spmv plan: null argument");
+  info[0] = p->parts;
+  info[1] = p->slabs;
+  info[2] = p->slab_width;
+  info[3] = p->row_weight;
+  info[4] = p->max_local_rows;
+  return 0;
+}

@@ -1,0 +1,49 @@
+"""Slab-tiled SpMV plan against the extended-precision oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from taco_amd import _lib, descriptors as D
+from taco_amd import workloads as W
+from tests import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+
+
+def run(A, x, parts=0, slab=0, unroll=0):
+    a = G.csr_desc(A)
+    plan = _lib.SpmvPlan(a, parts, slab)
+    y = torch.full((A.m,), float("nan"), dtype=torch.float64, device="cuda")
+    xd = G.dev(x)
+    plan.compute(D.dense(y), D.dense(xd), unroll)
+    torch.cuda.synchronize()
+    return y.cpu().numpy(), plan.info()
+
+
+CASES = [
+    ("powerlaw", lambda: W.csr_powerlaw(30000, 30000, seed=41), 0, 4096),
+    ("powerlaw_many_parts", lambda: W.csr_powerlaw(30000, 30000, seed=42), 3000, 1000),
+    ("one_slab", lambda: W.csr_powerlaw(20000, 20000, seed=43), 64, 0),
+    ("long_rows", lambda: W.csr_from_counts(10, 300000, np.array([0, 250000, 3, 0, 0, 40000, 1, 0, 0, 7], np.int32), 44), 16, 50000),
+    ("empty_rows", lambda: W.csr_from_counts(50000, 9000, W.uniform_counts(50000, 0, 2, 45), 45), 40, 2048),
+    ("no_entries", lambda: W.Csr(9, 4, np.zeros(10, np.int32), np.zeros(0, np.int32), np.zeros(0)), 4, 0),
+]
+
+
+@pytest.mark.parametrize("name,make,parts,slab", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("unroll", [1, 4, 8])
+def test_slab_spmv(name, make, parts, slab, unroll):
+    A = make()
+    print(f"slab spmv {name} m={A.m} nnz={A.nnz} parts={parts} slab={slab} unroll={unroll}", flush=True)
+    x = W.uniform(A.n, 5, W.STREAM_X)
+    y, info = run(A, x, parts, slab, unroll)
+    ye, cond = oracle.exact_spmv(A, x)
+    rep = oracle.error_report(y, ye, cond, 1e-12)
+    assert rep["ok"], (rep, info)
+
+
+def test_slab_plan_rejects_f32():
+    A = W.csr_fixed(100, 100, 3, 1)
+    with pytest.raises(_lib.TacoHipError, match="f64"):
+        _lib.SpmvPlan(G.csr_desc(A, "f32"))
