@@ -38,8 +38,10 @@ namespace {
 
 constexpr int kColBits = 18;
 constexpr uint32_t kColMask = (1u << kColBits) - 1;
-constexpr int kPlanThreads = 512;
-constexpr int kMaxLocalRows = 9728;  // 76 KB of f64 partials per workgroup
+constexpr int kPlanThreads = 256;
+// Row partials live in dynamic LDS sized by the plan's largest part; this
+// caps one part at 76 KB of f64 partials.
+constexpr int kMaxLocalRows = 9728;
 
 // Largest r in [0, m] with w*r + pos[r] <= d.
 __device__ int64_t weighted_row(const int32_t* pos, int64_t m, int64_t w,
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_slab_spmv(
     const uint32_t* __restrict__ ent, const double* __restrict__ val2,
     const double* __restrict__ x, double* __restrict__ y,
     int32_t* __restrict__ carry_row, double* __restrict__ carry_val) {
-  __shared__ double s_part[kMaxLocalRows];
+  extern __shared__ double s_part[];
   const int32_t p = blockIdx.x;
   const int tid = threadIdx.x;
   const int32_t r0 = part_row[p], r1 = part_row[p + 1];
@@ -273,7 +275,7 @@ extern "C" int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
   TH_REQUIRE(a.vt == VType::F64, "spmv plan: f64 values only");
   if (slab_width <= 0) slab_width = 1 << kColBits;
   TH_REQUIRE(slab_width <= (1 << kColBits), "spmv plan: slab width above 2^18");
-  if (parts <= 0) parts = 512;
+  if (parts <= 0) parts = 2048;
   const int32_t slabs = a.n == 0 ? 1 : (int32_t)(((int64_t)a.n + slab_width - 1) / slab_width);
   TH_REQUIRE(slabs <= 1024, "spmv plan: more than 1024 slabs");
   // Row weight: parts * rows_per_part must cover the rows with a margin,
@@ -363,9 +365,10 @@ extern "C" int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* p,
   hipStream_t st = current_stream();
   const double* xp = (const double*)xv.vals;
   double* yp = (double*)yv.vals;
+  const size_t lds = sizeof(double) * (size_t)(p->max_local_rows > 0 ? p->max_local_rows : 1);
 #define TH_SLAB(U)                                                          \
   case U:                                                                   \
-    k_slab_spmv<U><<<p->parts, kPlanThreads, 0, st>>>(                      \
+    k_slab_spmv<U><<<p->parts, kPlanThreads, lds, st>>>(                    \
         p->m, p->slabs, p->slab_width, p->part_row, p->off, p->ent,         \
         p->val2, xp, yp, p->carry_row, p->carry_val);                       \
     break;
