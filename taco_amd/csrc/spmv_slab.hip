@@ -183,21 +183,23 @@ __global__ __launch_bounds__(kPlanThreads) void k_slab_spmv(
   for (int32_t s = 0; s < slabs; ++s) {
     const int32_t e0 = o[s], e1 = o[s + 1];
     const double* xs = x + (int64_t)s * slab_width;
+    // Branch-free loads: out-of-range lanes re-read the chunk's last entry
+    // (a valid address) and drop their contribution, so the compiler issues
+    // all U streaming loads and then all U gathers back to back instead of
+    // waiting on each conditional load in turn.
     for (int32_t base = e0; base < e1; base += kPlanThreads * U) {
       uint32_t pk[U];
       double v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int32_t e = base + tid + u * kPlanThreads;
-        pk[u] = e < e1 ? stream_load(ent + e) : 0u;
-        v[u] = e < e1 ? stream_load(val2 + e) : 0.0;
+        const int32_t ec = e < e1 ? e : e1 - 1;
+        pk[u] = stream_load(ent + ec);
+        v[u] = stream_load(val2 + ec);
       }
       double xv[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int32_t e = base + tid + u * kPlanThreads;
-        xv[u] = e < e1 ? xs[pk[u] & kColMask] : 0.0;
-      }
+      for (int u = 0; u < U; ++u) xv[u] = xs[pk[u] & kColMask];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int32_t e = base + tid + u * kPlanThreads;
