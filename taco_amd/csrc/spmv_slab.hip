@@ -175,6 +175,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_slab_spmv(
   extern __shared__ double s_part[];
   const int32_t p = blockIdx.x;
   const int tid = threadIdx.x;
+  const int lane = tid % kWave;
   const int32_t r0 = part_row[p], r1 = part_row[p + 1];
   const int32_t nrows = r1 - r0 + 1;  // local rows, <= kMaxLocalRows by plan
   for (int32_t q = tid; q < nrows; q += kPlanThreads) s_part[q] = 0;
@@ -200,10 +201,20 @@ __global__ __launch_bounds__(kPlanThreads) void k_slab_spmv(
       double xv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) xv[u] = xs[pk[u] & kColMask];
+      // A wave holds 64 consecutive entries of the chunk, sorted by local
+      // row: reduce each row's run across the wave first, so only the last
+      // lane of each run touches LDS (one atomic per row run instead of one
+      // per entry).  Lanes past the chunk end get a key no row uses.
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int32_t e = base + tid + u * kPlanThreads;
-        if (e < e1) atomicAdd(&s_part[pk[u] >> kColBits], v[u] * xv[u]);
+        const bool valid = e < e1;
+        const int key = valid ? (int)(pk[u] >> kColBits) : INT_MAX;
+        const double prod = valid ? v[u] * xv[u] : 0.0;
+        const double run = wave_segmented_inclusive_sum(prod, key);
+        const int next = __shfl_down(key, 1, kWave);
+        const bool tail = (lane == kWave - 1) || next != key;
+        if (valid && tail) atomicAdd(&s_part[key], run);
       }
     }
   }
