@@ -175,7 +175,6 @@ __global__ __launch_bounds__(kPlanThreads) void k_slab_spmv(
   extern __shared__ double s_part[];
   const int32_t p = blockIdx.x;
   const int tid = threadIdx.x;
-  const int lane = tid % kWave;
   const int32_t r0 = part_row[p], r1 = part_row[p + 1];
   const int32_t nrows = r1 - r0 + 1;  // local rows, <= kMaxLocalRows by plan
   for (int32_t q = tid; q < nrows; q += kPlanThreads) s_part[q] = 0;
@@ -201,20 +200,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_slab_spmv(
       double xv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) xv[u] = xs[pk[u] & kColMask];
-      // A wave holds 64 consecutive entries of the chunk, sorted by local
-      // row: reduce each row's run across the wave first, so only the last
-      // lane of each run touches LDS (one atomic per row run instead of one
-      // per entry).  Lanes past the chunk end get a key no row uses.
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int32_t e = base + tid + u * kPlanThreads;
-        const bool valid = e < e1;
-        const int key = valid ? (int)(pk[u] >> kColBits) : INT_MAX;
-        const double prod = valid ? v[u] * xv[u] : 0.0;
-        const double run = wave_segmented_inclusive_sum(prod, key);
-        const int next = __shfl_down(key, 1, kWave);
-        const bool tail = (lane == kWave - 1) || next != key;
-        if (valid && tail) atomicAdd(&s_part[key], run);
+        if (e < e1) atomicAdd(&s_part[pk[u] >> kColBits], v[u] * xv[u]);
       }
     }
   }
@@ -288,7 +277,7 @@ extern "C" int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
   TH_REQUIRE(a.vt == VType::F64, "spmv plan: f64 values only");
   if (slab_width <= 0) slab_width = 1 << kColBits;
   TH_REQUIRE(slab_width <= (1 << kColBits), "spmv plan: slab width above 2^18");
-  if (parts <= 0) parts = 2048;
+  if (parts <= 0) parts = 1024;
   const int32_t slabs = a.n == 0 ? 1 : (int32_t)(((int64_t)a.n + slab_width - 1) / slab_width);
   TH_REQUIRE(slabs <= 1024, "spmv plan: more than 1024 slabs");
   // Row weight: parts * rows_per_part must cover the rows with a margin,
@@ -385,7 +374,7 @@ extern "C" int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* p,
         p->m, p->slabs, p->slab_width, p->part_row, p->off, p->ent,         \
         p->val2, xp, yp, p->carry_row, p->carry_val);                       \
     break;
-  switch (unroll <= 0 ? 4 : unroll) {
+  switch (unroll <= 0 ? 2 : unroll) {
     TH_SLAB(1) TH_SLAB(2) TH_SLAB(4) TH_SLAB(8)
     default:
       return fail("spmv plan: unroll %d unsupported", unroll);
