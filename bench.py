@@ -143,6 +143,20 @@ def gpu_setup():
     return torch, L, _lib
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration
+    recorded with it), or None when no summary covers the kernel."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        import json as _json
+        with open(path) as f:
+            data = _json.load(f)
+        return data["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def sched(_lib, algo, block=0, items=0, group=0, rows=0):
     s = _lib.Schedule()
     s.algo, s.block_size, s.items_per_thread = algo, block, items
@@ -164,22 +178,32 @@ def bench_spmv(args, A, x, name, workload_desc):
     xd = torch.from_numpy(x).cuda()
     y = torch.empty(A.m, dtype=torch.float64, device="cuda")
     a, xdsc, ydsc = D.csr(pos, crd, vals, A.shape), D.dense(xd), D.dense(y)
-    s_main = sched(_lib, _lib.MERGE_PATH, 256, args.items)
+    s_merge = sched(_lib, _lib.MERGE_PATH, 256, args.items)
     s_default = sched(_lib, _lib.THREAD_PER_ROW, 256)
 
     def step(s):
         return lambda: _lib.check(L.taco_hip_spmv_compute(ydsc.ptr, a.ptr, xdsc.ptr, s))
 
     timer = Timer(torch)
-    log(f"gpu: {args.warmup} warmup + {args.steps} steps ...")
-    ms, wall_ms = timer.run(step(s_main), args.steps, args.warmup)
+    # Slab-tiled plan: a one-time storage conversion of A, timed separately.
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan = _lib.SpmvPlan(a, args.parts, 0)
+    torch.cuda.synchronize()
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    log(f"gpu: plan {plan_ms:.1f} ms {plan.info()}; {args.warmup} warmup + {args.steps} steps ...")
+    ms, wall_ms = timer.run(lambda: plan.compute(ydsc, xdsc, args.unroll), args.steps, args.warmup)
     y_host = y.cpu().numpy()
-    ms_default, _ = timer.run(step(s_default), max(5, args.steps // 5), 2)
+    ms_merge, _ = timer.run(step(s_merge), max(10, args.steps // 2), 3)
+    y_merge = y.cpu().numpy()
+    ms_default, _ = timer.run(step(s_default), max(5, args.steps // 10), 2)
     nbytes = W.algorithmic_bytes_spmv(A.m, A.n, A.nnz)
     gbs = nbytes / (ms * 1e-3) / 1e9
     check = None
     if y_ref is not None:
-        check = float(np.abs(y_host - y_ref).max() / max(np.abs(y_ref).max(), 1e-300))
+        scale = max(np.abs(y_ref).max(), 1e-300)
+        check = {"slab_plan": float(np.abs(y_host - y_ref).max() / scale),
+                 "merge_path": float(np.abs(y_merge - y_ref).max() / scale)}
     cpu = None
     if cpu_ts:
         cms = trimmed_mean(cpu_ts)
@@ -187,6 +211,7 @@ def bench_spmv(args, A, x, name, workload_desc):
                "kind": kind, "ms": cms, "median_ms": float(np.median(cpu_ts)),
                "sample": f"whole workload, {len(cpu_ts)} timed compute() calls "
                          f"(10% trimmed mean), JIT excluded, inputs in host DRAM"}
+    traffic = profiled_traffic("k_slab_spmv")
     return {
         "metric": "SpMV HBM GB/s (% of 8 TB/s), SpMM GFLOP/s: learned vs default schedule at tol",
         "value": gbs, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -194,16 +219,20 @@ def bench_spmv(args, A, x, name, workload_desc):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seed-generated)",
         "config": {"workload": name, "desc": workload_desc, "m": A.m, "n": A.n,
-                   "nnz": A.nnz, "schedule": f"merge_path items_per_thread={args.items}",
+                   "nnz": A.nnz, "schedule": f"slab-tiled plan parts={plan.info()['parts']} "
+                                             f"slab_width={plan.info()['slab_width']} unroll={args.unroll}",
                    "parallelism": "single GPU"},
         "pct_of_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS,
         "gflops": 2.0 * A.nnz / (ms * 1e-3) / 1e9,
         "nnz_per_s": A.nnz / (ms * 1e-3),
+        "plan_build_ms": plan_ms,
+        "merge_path": {"ms": ms_merge, "gbs": nbytes / (ms_merge * 1e-3) / 1e9,
+                       "note": "deterministic CSR merge-path kernel, no plan"},
         "default_schedule": {"schedule": "thread_per_row (taco split(i,256))",
                              "ms": ms_default, "speedup_vs_default": ms_default / ms},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv_merge + k_spmv_merge_fixup (one compute call)",
+                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_slab_spmv + k_slab_fixup (one compute call)",
                      "algorithmic_bytes": nbytes},
         "check_vs_cpu_reference": check,
         "cpu_baseline": cpu,
@@ -361,6 +390,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="spmv_powerlaw")
     ap.add_argument("--items", type=int, default=8, help="SpMV merge items per thread")
+    ap.add_argument("--parts", type=int, default=1024, help="slab plan parts")
+    ap.add_argument("--unroll", type=int, default=2, help="slab plan entries per thread per step")
     ap.add_argument("--spmm-items", type=int, default=32)
     ap.add_argument("--mfma", type=int, default=1)
     ap.add_argument("--cpu-reps", type=int, default=10)
