@@ -175,6 +175,40 @@ void taco_hip_spmv_plan_destroy(taco_hip_spmv_plan* plan);
 /* info[0..4] = parts, slabs, slab width, row weight, max rows in a part. */
 int taco_hip_spmv_plan_info(const taco_hip_spmv_plan* plan, int64_t* info);
 
+/* ======================================================================
+ * Batched uneven SpMV / SpMM (f32): many CSR instances in one persistent
+ * launch.  No reference counterpart (taco launches one kernel per tensor).
+ * Instances are concatenated: instance i's pos (m_i + 1 int32 local
+ * offsets), crd (local columns) and vals follow instance i-1's; x/B and y/C
+ * segments likewise (B, C row-major with width K).  sched[i] is the lanes
+ * per row: SpMV any power of two 1..64, SpMM 8..64 with K <= 4 * lanes.
+ * The device buffers are borrowed and must outlive the batch.
+ * ==================================================================== */
+typedef struct taco_hip_batch taco_hip_batch;
+/* op 0 = SpMV (in = x, out = y), 1 = SpMM (in = B, out = C, width K). */
+int taco_hip_batch_create(int32_t op, int32_t ninst, const int32_t* m,
+                          const int32_t* n, const int64_t* nnz,
+                          const int32_t* sched, int32_t rows_per_task,
+                          int32_t K, const int32_t* pos, const int32_t* crd,
+                          const float* vals, const float* in, float* out,
+                          taco_hip_batch** result);
+int taco_hip_batch_set_schedules(taco_hip_batch* b, const int32_t* sched);
+/* One launch (queue reset + persistent kernel) on the current stream. */
+int taco_hip_batch_run(taco_hip_batch* b);
+/* One launch recording each instance's summed task time (ns); synchronous. */
+int taco_hip_batch_run_timed(taco_hip_batch* b, double* times_ns);
+/* Capture (queue reset, persistent kernel) into a hipGraph; then replay it
+ * on the current stream with taco_hip_batch_launch_graph. */
+int taco_hip_batch_capture(taco_hip_batch* b);
+int taco_hip_batch_launch_graph(taco_hip_batch* b);
+void taco_hip_batch_destroy(taco_hip_batch* b);
+/* info[0..2] = instances, tasks, persistent workgroups. */
+int taco_hip_batch_info(const taco_hip_batch* b, int64_t* info);
+/* Device f32 values U[lo, hi): element i is taco_hip_gen_uniform_f64's
+ * element i of the same (seed, stream), rounded to f32. */
+int taco_hip_gen_uniform_f32_device(int64_t n, uint64_t seed, uint64_t stream,
+                                    double lo, double hi, float* out);
+
 /* Device exclusive scan: pos[0] = 0, pos[i+1] = pos[i] + counts[i], on the
  * current stream.  Returns non-zero (and records an error) if the total
  * exceeds the int32 index range; synchronises the stream to check. */

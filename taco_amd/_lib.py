@@ -88,6 +88,18 @@ _SIGS = {
     "taco_hip_spmv_plan_compute": (c_int, [c_void_p, P_T, P_T, c_int32]),
     "taco_hip_spmv_plan_destroy": (None, [c_void_p]),
     "taco_hip_spmv_plan_info": (c_int, [c_void_p, POINTER(c_int64)]),
+    "taco_hip_batch_create": (c_int, [c_int32, c_int32, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_int32, c_int32, c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_void_p, POINTER(c_void_p)]),
+    "taco_hip_batch_set_schedules": (c_int, [c_void_p, c_void_p]),
+    "taco_hip_batch_run": (c_int, [c_void_p]),
+    "taco_hip_batch_run_timed": (c_int, [c_void_p, c_void_p]),
+    "taco_hip_batch_capture": (c_int, [c_void_p]),
+    "taco_hip_batch_launch_graph": (c_int, [c_void_p]),
+    "taco_hip_batch_destroy": (None, [c_void_p]),
+    "taco_hip_batch_info": (c_int, [c_void_p, POINTER(c_int64)]),
+    "taco_hip_gen_uniform_f32_device": (c_int, [c_int64, c_uint64, c_uint64, c_double,
+                                                c_double, c_void_p]),
     "taco_hip_device_exclusive_scan": (c_int, [c_void_p, c_int32, c_void_p]),
     "taco_hip_workspace_get": (c_void_p, [c_size_t, c_int]),
     "taco_hip_generate": (c_void_p, [c_void_p, c_int32]),

@@ -1,0 +1,124 @@
+"""Exhaustive autotuner for the hot-path schedules.
+
+Two forms:
+  * ``tune_batch`` — per-instance schedules of a batched launch.  Every
+    candidate lane count runs the whole batch once with per-instance device
+    timing (the instances' summed task times), so the exhaustive search over
+    C candidates costs C timed launches regardless of the instance count.
+    Each candidate's output is first checked against the reference candidate
+    (thread per row, the sequential summation order); a candidate whose
+    result differs beyond the value tolerance is never selected.
+  * ``tune_spmv`` / ``tune_spmm`` — one matrix, every schedule in the
+    candidate space timed with HIP events; returns the ranking.
+
+The candidate spaces are the schedule families the kernels implement; they
+are also the classes the learned selector predicts.
+"""
+import numpy as np
+
+from . import _lib
+from . import batch as BT
+
+
+def _sched(algo, block=0, items=0, group=0, rows=0):
+    s = _lib.Schedule()
+    s.algo, s.block_size, s.items_per_thread = algo, block, items
+    s.group_size, s.rows_per_group = group, rows
+    return s
+
+
+# The SpMV schedule space (also the selector's SpMV classes).
+SPMV_CANDIDATES = [
+    ("thread_per_row", _sched(_lib.THREAD_PER_ROW, 256)),
+    ("group2", _sched(_lib.GROUP_PER_ROW, 256, group=2)),
+    ("group4", _sched(_lib.GROUP_PER_ROW, 256, group=4)),
+    ("group8", _sched(_lib.GROUP_PER_ROW, 256, group=8)),
+    ("group16", _sched(_lib.GROUP_PER_ROW, 256, group=16)),
+    ("group32", _sched(_lib.GROUP_PER_ROW, 256, group=32)),
+    ("group64", _sched(_lib.GROUP_PER_ROW, 256, group=64)),
+    ("nnz_split4", _sched(_lib.NNZ_SPLIT, 256, items=4)),
+    ("nnz_split8", _sched(_lib.NNZ_SPLIT, 256, items=8)),
+    ("nnz_split16", _sched(_lib.NNZ_SPLIT, 256, items=16)),
+    ("merge4", _sched(_lib.MERGE_PATH, 256, items=4)),
+    ("merge8", _sched(_lib.MERGE_PATH, 256, items=8)),
+    ("merge16", _sched(_lib.MERGE_PATH, 256, items=16)),
+]
+
+# The SpMM schedule space for a dense width K (lane teams of up to 64).
+def spmm_candidates(K):
+    g = 8
+    while g < K and g < 64:
+        g *= 2
+    out = [("thread_per_row", _sched(_lib.THREAD_PER_ROW, 256))]
+    for rows in (1, 2, 4):
+        out.append((f"rows{rows}", _sched(_lib.GROUP_PER_ROW, 256, group=g, rows=rows)))
+    for items in (4, 8, 16):
+        out.append((f"nnz_split{items}", _sched(_lib.NNZ_SPLIT, 256, items=items, group=g)))
+    for items in (8, 16, 32, 64):
+        out.append((f"merge{items}", _sched(_lib.MERGE_PATH, 256, items=items, group=g)))
+    return out
+
+
+def _time(fn, reps, warmup=2):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune_spmv(a_desc, x_desc, y_desc, reps=5, candidates=None):
+    """Times every SpMV schedule on one matrix; returns [(name, ms)] sorted."""
+    L = _lib.lib()
+    out = []
+    for name, s in candidates or SPMV_CANDIDATES:
+        fn = lambda s=s: _lib.check(L.taco_hip_spmv_compute(y_desc.ptr, a_desc.ptr, x_desc.ptr, s))
+        out.append((name, _time(fn, reps)))
+    return sorted(out, key=lambda t: t[1])
+
+
+def tune_spmm(a_desc, b_desc, c_desc, K, reps=3, candidates=None):
+    L = _lib.lib()
+    out = []
+    for name, s in candidates or spmm_candidates(K):
+        fn = lambda s=s: _lib.check(L.taco_hip_spmm_compute(c_desc.ptr, a_desc.ptr, b_desc.ptr, s))
+        out.append((name, _time(fn, reps)))
+    return sorted(out, key=lambda t: t[1])
+
+
+def tune_batch(wl, op, reps=3, rtol=1e-4, check=True):
+    """Exhaustive per-instance schedule search for a batched launch.
+
+    Returns (best lanes per instance, times[candidate, instance] in ns,
+    candidate lane counts, candidates rejected by the output check)."""
+    import torch
+    cands = list(BT.SPMV_LANES if op == 0 else BT.SPMM_LANES)
+    ref_lanes = cands[0]
+    b = BT.Batch(wl, op, np.full(wl.ninst, ref_lanes, np.int32))
+    out = wl.y if op == 0 else wl.C
+    b.run()
+    torch.cuda.synchronize()
+    ref = out.clone()
+    scale = float(ref.abs().max().item()) if ref.numel() else 1.0
+    times = np.zeros((len(cands), wl.ninst))
+    rejected = []
+    for ci, lanes in enumerate(cands):
+        b.set_schedules(np.full(wl.ninst, lanes, np.int32))
+        b.run_timed()  # warm-up launch
+        ts = [b.run_timed() for _ in range(reps)]
+        times[ci] = np.median(np.stack(ts), axis=0)
+        if check:
+            diff = float((out - ref).abs().max().item())
+            if diff > rtol * max(scale, 1e-30):
+                rejected.append(lanes)
+                times[ci] = np.inf
+    best = np.array(cands, np.int32)[np.argmin(times, axis=0)]
+    b.set_schedules(best)
+    return best, times, cands, rejected

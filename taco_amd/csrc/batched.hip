@@ -1,0 +1,400 @@
+// Batched SpMV / SpMM over many uneven CSR instances in one persistent
+// launch (no reference counterpart; taco compiles one kernel per statement
+// and launches it per tensor).
+//
+// Storage: the instances are concatenated.  Instance i owns
+//   pos[pos_off_i .. pos_off_i + m_i]           (local entry offsets, int32)
+//   crd/vals[ent_off_i .. ent_off_i + nnz_i)      (local column ids, f32)
+//   x[col_off_i .. + n_i), y[row_off_i .. + m_i)  (SpMV)
+//   B[col_off_i*K ..], C[row_off_i*K ..]          (SpMM, row-major, width K)
+// with 64-bit offsets, so the batch may exceed 2^31 entries while indices
+// inside an instance stay int32.
+//
+// Work: the host splits every instance into tasks of `rows_per_task` rows;
+// a persistent grid (a few workgroups per CU) pulls tasks from a queue
+// counter.  Each instance carries its own schedule:
+//   SpMV: G lanes per row, G in {1, 2, 4, ..., 64} (group-per-row with a
+//         shuffle-tree reduction; G = 1 is thread-per-row)
+//   SpMM: L lanes per row along k (K / L columns per lane), L in {8,16,32,64}
+// Optional per-instance timing: every task adds its duration (the 100 MHz
+// s_memrealtime clock) into the instance's counter, which the autotuner uses
+// to pick the best schedule per instance from one launch per candidate.
+// The queue counter is reset by a one-thread kernel, so the pair (reset,
+// batch) can be captured into a hipGraph and replayed.
+
+#include <vector>
+
+#include "common.h"
+#include "device.h"
+
+namespace taco_hip {
+namespace {
+
+struct Inst {
+  int64_t pos_off, ent_off, row_off, col_off;
+  int32_t m, n, sched, pad;
+};
+
+struct Task {
+  int32_t inst, r0, r1, pad;
+};
+
+constexpr int kBatchThreads = 256;
+
+__device__ __forceinline__ uint64_t realtime() {
+  return __builtin_amdgcn_s_memrealtime();
+}
+
+__global__ void k_reset_queue(uint32_t* q) { q[0] = 0; }
+
+template <bool TIMING>
+__global__ __launch_bounds__(kBatchThreads) void k_batch_spmv(
+    int32_t ntasks, const Task* __restrict__ tasks, const Inst* __restrict__ insts,
+    const int32_t* __restrict__ pos, const int32_t* __restrict__ crd,
+    const float* __restrict__ vals, const float* __restrict__ x,
+    float* __restrict__ y, uint32_t* __restrict__ queue,
+    unsigned long long* __restrict__ inst_time) {
+  __shared__ int32_t s_task;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) s_task = (int32_t)atomicAdd(queue, 1u);
+    __syncthreads();
+    const int32_t t = s_task;
+    __syncthreads();
+    if (t >= ntasks) break;  // every thread of the workgroup sees the same t
+    uint64_t t0 = 0;
+    if (TIMING) t0 = realtime();
+    const Task tk = tasks[t];
+    const Inst in = insts[tk.inst];
+    const int G = in.sched;  // lanes per row, power of two <= 64
+    const int lane = tid & (G - 1);
+    const int rows_per_pass = kBatchThreads / G;
+    const int32_t* ip = pos + in.pos_off;
+    const int32_t* ic = crd + in.ent_off;
+    const float* iv = vals + in.ent_off;
+    const float* ix = x + in.col_off;
+    float* iy = y + in.row_off;
+    for (int32_t r = tk.r0 + tid / G; r < tk.r1; r += rows_per_pass) {
+      float acc = 0.f;
+      const int32_t end = ip[r + 1];
+      for (int32_t q = ip[r] + lane; q < end; q += G)
+        acc += stream_load(iv + q) * ix[stream_load(ic + q)];
+      for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+      if (lane == 0) iy[r] = acc;
+    }
+    if (TIMING) {
+      __syncthreads();
+      if (tid == 0) atomicAdd(inst_time + tk.inst, (unsigned long long)(realtime() - t0));
+    }
+  }
+}
+
+template <bool TIMING>
+__global__ __launch_bounds__(kBatchThreads) void k_batch_spmm(
+    int32_t ntasks, int32_t K, const Task* __restrict__ tasks,
+    const Inst* __restrict__ insts, const int32_t* __restrict__ pos,
+    const int32_t* __restrict__ crd, const float* __restrict__ vals,
+    const float* __restrict__ B, float* __restrict__ C,
+    uint32_t* __restrict__ queue, unsigned long long* __restrict__ inst_time) {
+  __shared__ int32_t s_task;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) s_task = (int32_t)atomicAdd(queue, 1u);
+    __syncthreads();
+    const int32_t t = s_task;
+    __syncthreads();
+    if (t >= ntasks) break;
+    uint64_t t0 = 0;
+    if (TIMING) t0 = realtime();
+    const Task tk = tasks[t];
+    const Inst in = insts[tk.inst];
+    const int L = in.sched;  // lanes per row along k
+    const int lane = tid & (L - 1);
+    const int rows_per_pass = kBatchThreads / L;
+    const int32_t* ip = pos + in.pos_off;
+    const int32_t* ic = crd + in.ent_off;
+    const float* iv = vals + in.ent_off;
+    const float* iB = B + in.col_off * (int64_t)K;
+    float* iC = C + in.row_off * (int64_t)K;
+    for (int32_t r = tk.r0 + tid / L; r < tk.r1; r += rows_per_pass) {
+      float acc[4] = {0.f, 0.f, 0.f, 0.f};  // K <= 4 * L
+      const int32_t end = ip[r + 1];
+      for (int32_t q = ip[r]; q < end; ++q) {
+        const float a = stream_load(iv + q);
+        const float* brow = iB + (int64_t)stream_load(ic + q) * K;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int k = lane + c * L;
+          if (k < K) acc[c] += a * brow[k];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int k = lane + c * L;
+        if (k < K) iC[(int64_t)r * K + k] = acc[c];
+      }
+    }
+    if (TIMING) {
+      __syncthreads();
+      if (tid == 0) atomicAdd(inst_time + tk.inst, (unsigned long long)(realtime() - t0));
+    }
+  }
+}
+
+// 32-bit columns of a generated f32 value stream: U[lo, hi) from the same
+// splitmix64 counter formula as taco_hip_gen_uniform_f64, rounded to f32.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_gen_uniform_f32(int64_t n, uint64_t key,
+                                                         double lo, double span,
+                                                         float* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double u = (double)(mix64(key + (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0);
+    out[i] = (float)(lo + span * u);
+  }
+}
+
+}  // namespace
+}  // namespace taco_hip
+
+using namespace taco_hip;
+
+struct taco_hip_batch {
+  int32_t ninst = 0, ntasks = 0, K = 0;
+  int op = 0;  // 0 SpMV, 1 SpMM
+  Inst* insts = nullptr;
+  Task* tasks = nullptr;
+  uint32_t* queue = nullptr;
+  unsigned long long* inst_time = nullptr;
+  const int32_t* pos = nullptr;
+  const int32_t* crd = nullptr;
+  const float* vals = nullptr;
+  const float* in = nullptr;  // x or B
+  float* out = nullptr;       // y or C
+  int grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<Inst> host_insts;
+};
+
+namespace {
+void batch_free(taco_hip_batch* b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void* q : {(void*)b->insts, (void*)b->tasks, (void*)b->queue, (void*)b->inst_time})
+    if (q) (void)hipFree(q);
+  delete b;
+}
+
+int launch_batch(taco_hip_batch* b, hipStream_t st, bool timing) {
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, st, b->queue);
+  if (b->op == 0) {
+    if (timing)
+      hipLaunchKernelGGL(k_batch_spmv<true>, dim3(b->grid), dim3(kBatchThreads), 0, st,
+                         b->ntasks, b->tasks, b->insts, b->pos, b->crd, b->vals, b->in,
+                         b->out, b->queue, b->inst_time);
+    else
+      hipLaunchKernelGGL(k_batch_spmv<false>, dim3(b->grid), dim3(kBatchThreads), 0, st,
+                         b->ntasks, b->tasks, b->insts, b->pos, b->crd, b->vals, b->in,
+                         b->out, b->queue, b->inst_time);
+  } else {
+    if (timing)
+      hipLaunchKernelGGL(k_batch_spmm<true>, dim3(b->grid), dim3(kBatchThreads), 0, st,
+                         b->ntasks, b->K, b->tasks, b->insts, b->pos, b->crd, b->vals,
+                         b->in, b->out, b->queue, b->inst_time);
+    else
+      hipLaunchKernelGGL(k_batch_spmm<false>, dim3(b->grid), dim3(kBatchThreads), 0, st,
+                         b->ntasks, b->K, b->tasks, b->insts, b->pos, b->crd, b->vals,
+                         b->in, b->out, b->queue, b->inst_time);
+  }
+  TH_LAUNCH_CHECK("batched kernel");
+  return 0;
+}
+
+bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+}  // namespace
+
+extern "C" {
+
+int taco_hip_batch_create(int32_t op, int32_t ninst, const int32_t* m,
+                          const int32_t* n, const int64_t* nnz,
+                          const int32_t* sched, int32_t rows_per_task,
+                          int32_t K, const int32_t* pos, const int32_t* crd,
+                          const float* vals, const float* in, float* out,
+                          taco_hip_batch** result) {
+  TH_REQUIRE(result, "batch: null output");
+  *result = nullptr;
+  TH_REQUIRE(op == 0 || op == 1, "batch: op must be 0 (SpMV) or 1 (SpMM)");
+  TH_REQUIRE(ninst >= 0 && m && n && nnz && sched, "batch: bad instance arrays");
+  TH_REQUIRE(rows_per_task > 0, "batch: rows_per_task must be positive");
+  TH_REQUIRE(op == 0 || (K > 0 && K <= 256), "batch: SpMM width must be in 1..256");
+  auto* b = new taco_hip_batch();
+  b->op = op;
+  b->ninst = ninst;
+  b->K = K;
+  b->pos = pos;
+  b->crd = crd;
+  b->vals = vals;
+  b->in = in;
+  b->out = out;
+  std::vector<Task> tasks;
+  b->host_insts.resize((size_t)ninst);
+  int64_t po = 0, eo = 0, ro = 0, co = 0;
+  for (int32_t i = 0; i < ninst; ++i) {
+    const int s = sched[i];
+    if (!pow2(s) || s > 64 || (op == 1 && (s < 8 || K > 4 * s))) {
+      batch_free(b);
+      return fail("batch: instance %d has schedule %d, not a valid lane count", i, s);
+    }
+    if (m[i] < 0 || n[i] < 0 || nnz[i] < 0) {
+      batch_free(b);
+      return fail("batch: instance %d has negative sizes", i);
+    }
+    Inst& in_ = b->host_insts[(size_t)i];
+    in_.pos_off = po;
+    in_.ent_off = eo;
+    in_.row_off = ro;
+    in_.col_off = co;
+    in_.m = m[i];
+    in_.n = n[i];
+    in_.sched = s;
+    in_.pad = 0;
+    for (int32_t r = 0; r < m[i]; r += rows_per_task)
+      tasks.push_back(Task{i, r, r + rows_per_task < m[i] ? r + rows_per_task : m[i], 0});
+    po += (int64_t)m[i] + 1;
+    eo += nnz[i];
+    ro += m[i];
+    co += n[i];
+  }
+  if (tasks.size() > (size_t)INT32_MAX) {
+    batch_free(b);
+    return fail("batch: too many tasks");
+  }
+  b->ntasks = (int32_t)tasks.size();
+  hipStream_t st = current_stream();
+  if (hipMalloc((void**)&b->insts, sizeof(Inst) * (ninst > 0 ? ninst : 1)) ||
+      hipMalloc((void**)&b->tasks, sizeof(Task) * (tasks.empty() ? 1 : tasks.size())) ||
+      hipMalloc((void**)&b->queue, 64) ||
+      hipMalloc((void**)&b->inst_time, sizeof(unsigned long long) * (ninst > 0 ? ninst : 1))) {
+    batch_free(b);
+    return fail("batch: device allocation failed");
+  }
+  if ((ninst > 0 && hipMemcpyAsync(b->insts, b->host_insts.data(), sizeof(Inst) * ninst,
+                                   hipMemcpyHostToDevice, st)) ||
+      (!tasks.empty() && hipMemcpyAsync(b->tasks, tasks.data(), sizeof(Task) * tasks.size(),
+                                        hipMemcpyHostToDevice, st)) ||
+      hipStreamSynchronize(st)) {
+    batch_free(b);
+    return fail("batch: upload failed");
+  }
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  b->grid = cus * 8;
+  if (b->grid > b->ntasks) b->grid = b->ntasks > 0 ? b->ntasks : 1;
+  *result = b;
+  return 0;
+}
+
+// Replaces the per-instance schedules (same validity rules as create).
+int taco_hip_batch_set_schedules(taco_hip_batch* b, const int32_t* sched) {
+  TH_REQUIRE(b && sched, "batch: null argument");
+  for (int32_t i = 0; i < b->ninst; ++i) {
+    const int s = sched[i];
+    TH_REQUIRE(pow2(s) && s <= 64 && (b->op == 0 || (s >= 8 && b->K <= 4 * s)),
+               "batch: instance %d has schedule %d, not a valid lane count", i, s);
+    b->host_insts[(size_t)i].sched = s;
+  }
+  if (b->ninst > 0)
+    TH_CHECK(hipMemcpyAsync(b->insts, b->host_insts.data(), sizeof(Inst) * b->ninst,
+                            hipMemcpyHostToDevice, current_stream()));
+  TH_CHECK(hipStreamSynchronize(current_stream()));
+  return 0;
+}
+
+int taco_hip_batch_run(taco_hip_batch* b) {
+  TH_REQUIRE(b, "batch: null");
+  if (b->ntasks == 0) return 0;
+  if (int rc = launch_batch(b, current_stream(), false)) return rc;
+  return finish("taco_hip_batch_run");
+}
+
+// One launch with per-instance timing; times[i] receives instance i's summed
+// task durations in nanoseconds (synchronous).
+int taco_hip_batch_run_timed(taco_hip_batch* b, double* times_ns) {
+  TH_REQUIRE(b && times_ns, "batch: null argument");
+  hipStream_t st = current_stream();
+  if (b->ninst == 0) return 0;
+  TH_CHECK(hipMemsetAsync(b->inst_time, 0, sizeof(unsigned long long) * b->ninst, st));
+  if (b->ntasks > 0)
+    if (int rc = launch_batch(b, st, true)) return rc;
+  std::vector<unsigned long long> t((size_t)b->ninst);
+  TH_CHECK(hipMemcpyAsync(t.data(), b->inst_time, sizeof(unsigned long long) * b->ninst,
+                          hipMemcpyDeviceToHost, st));
+  TH_CHECK(hipStreamSynchronize(st));
+  for (int32_t i = 0; i < b->ninst; ++i) times_ns[i] = (double)t[(size_t)i] * 10.0;  // 100 MHz
+  return 0;
+}
+
+// Captures (queue reset, batch kernel) into a hipGraph on a private stream.
+int taco_hip_batch_capture(taco_hip_batch* b) {
+  TH_REQUIRE(b, "batch: null");
+  if (b->exec) return 0;
+  hipStream_t cap;
+  TH_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  int rc = 0;
+  if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = fail("batch: begin capture failed");
+  } else {
+    if (b->ntasks > 0) rc = launch_batch(b, cap, false);
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(cap, &g) != hipSuccess && !rc) rc = fail("batch: end capture failed");
+    if (!rc) {
+      b->graph = g;
+      if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess)
+        rc = fail("batch: graph instantiate failed");
+    } else if (g) {
+      (void)hipGraphDestroy(g);
+    }
+  }
+  (void)hipStreamDestroy(cap);
+  return rc;
+}
+
+int taco_hip_batch_launch_graph(taco_hip_batch* b) {
+  TH_REQUIRE(b && b->exec, "batch: no captured graph");
+  TH_CHECK(hipGraphLaunch(b->exec, current_stream()));
+  return finish("taco_hip_batch_launch_graph");
+}
+
+void taco_hip_batch_destroy(taco_hip_batch* b) { batch_free(b); }
+
+int taco_hip_batch_info(const taco_hip_batch* b, int64_t* info) {
+  TH_REQUIRE(b && info, "batch: null argument");
+  info[0] = b->ninst;
+  info[1] = b->ntasks;
+  info[2] = b->grid;
+  return 0;
+}
+
+// f32 values U[lo, hi) on the device with the counter formula of
+// taco_hip_gen_uniform_f64 (value i = lo + (hi-lo) * unit(i)), rounded.
+int taco_hip_gen_uniform_f32_device(int64_t n, uint64_t seed, uint64_t stream,
+                                    double lo, double hi, float* out) {
+  if (n <= 0) return 0;
+  const uint64_t key = mix64(seed ^ mix64(stream));
+  int64_t grid = (n + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(k_gen_uniform_f32, dim3((unsigned)grid), dim3(256), 0,
+                     current_stream(), n, key, lo, hi - lo, out);
+  return finish("taco_hip_gen_uniform_f32_device");
+}
+
+}  // extern "C"
