@@ -92,8 +92,12 @@ def test_batched_equals_unbatched(wl):
     full = wl.y.clone()
     single = BT.BatchedWorkload(ninst=1, rmin=int(wl.m[0]), rmax=int(wl.m[0]), seed=wl.seed,
                                 width=32, keep_host=True)
-    # instance 0 of both workloads has the same seed-derived pattern
+    # instance 0 of both workloads has the same seed-derived pattern and values
     assert np.array_equal(single.instance(0)[0].crd, wl.instance(0)[0].crd)
+    sb = BT.Batch(single, 0, np.full(1, 8, np.int32))
+    sb.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(single.y_of(0), full[:int(wl.m[0])].cpu().numpy().astype(np.float64))
 
 
 def test_autotune_batch(wl):
