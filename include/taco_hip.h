@@ -195,8 +195,13 @@ int taco_hip_batch_create(int32_t op, int32_t ninst, const int32_t* m,
 int taco_hip_batch_set_schedules(taco_hip_batch* b, const int32_t* sched);
 /* One launch (queue reset + persistent kernel) on the current stream. */
 int taco_hip_batch_run(taco_hip_batch* b);
-/* One launch recording each instance's summed task time (ns); synchronous. */
-int taco_hip_batch_run_timed(taco_hip_batch* b, double* times_ns);
+/* One launch over instances [i0, i1) only (their tasks are contiguous). */
+int taco_hip_batch_run_range(taco_hip_batch* b, int32_t i0, int32_t i1);
+/* Autotuner timing: each instance alone, `reps` launches back to back
+ * between two HIP events on the current stream; times_ns[i] = mean time of
+ * one launch over instance i.  Synchronous (one wait at the end). */
+int taco_hip_batch_time_instances(taco_hip_batch* b, int32_t reps,
+                                  double* times_ns);
 /* Capture (queue reset, persistent kernel) into a hipGraph; then replay it
  * on the current stream with taco_hip_batch_launch_graph. */
 int taco_hip_batch_capture(taco_hip_batch* b);

@@ -1,6 +1,7 @@
 """ctypes binding of libtaco_hip.so (the C ABI declared in include/taco_hip.h).
 
-The library is built in-tree by ``taco_amd.build.build()``.  There is no
+The library is built in-tree by ``make -C taco_amd/csrc`` (which the
+repository's ``__graft_entry__.build()`` runs).  There is no
 fallback: if the shared object is missing or does not load, every entry
 point raises, so a run can never silently compute on the CPU.
 """
@@ -93,7 +94,8 @@ _SIGS = {
                                       c_void_p, c_void_p, c_void_p, POINTER(c_void_p)]),
     "taco_hip_batch_set_schedules": (c_int, [c_void_p, c_void_p]),
     "taco_hip_batch_run": (c_int, [c_void_p]),
-    "taco_hip_batch_run_timed": (c_int, [c_void_p, c_void_p]),
+    "taco_hip_batch_run_range": (c_int, [c_void_p, c_int32, c_int32]),
+    "taco_hip_batch_time_instances": (c_int, [c_void_p, c_int32, c_void_p]),
     "taco_hip_batch_capture": (c_int, [c_void_p]),
     "taco_hip_batch_launch_graph": (c_int, [c_void_p]),
     "taco_hip_batch_destroy": (None, [c_void_p]),
@@ -189,8 +191,7 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise TacoHipError(
-            f"{LIB_PATH} is missing: build it with taco_amd.build.build() "
-            "(make -C taco_amd/csrc)")
+            f"{LIB_PATH} is missing: build it with make -C taco_amd/csrc")
     handle = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
     for name, (res, args) in _SIGS.items():
         fn = getattr(handle, name)

@@ -1,13 +1,13 @@
 """Exhaustive autotuner for the hot-path schedules.
 
 Two forms:
-  * ``tune_batch`` — per-instance schedules of a batched launch.  Every
-    candidate lane count runs the whole batch once with per-instance device
-    timing (the instances' summed task times), so the exhaustive search over
-    C candidates costs C timed launches regardless of the instance count.
-    Each candidate's output is first checked against the reference candidate
-    (thread per row, the sequential summation order); a candidate whose
-    result differs beyond the value tolerance is never selected.
+  * ``tune_batch`` — per-instance schedules of a batched launch.  For every
+    candidate lane count, each instance is launched alone ``reps`` times
+    between two HIP events (all instances queued back to back, one wait per
+    candidate).  Each candidate's whole-batch output is first checked
+    against the reference candidate (one lane per row, the sequential
+    summation order); a candidate whose result differs beyond the value
+    tolerance is never selected.
   * ``tune_spmv`` / ``tune_spmm`` — one matrix, every schedule in the
     candidate space timed with HIP events; returns the ranking.
 
@@ -111,9 +111,9 @@ def tune_batch(wl, op, reps=3, rtol=1e-4, check=True):
     rejected = []
     for ci, lanes in enumerate(cands):
         b.set_schedules(np.full(wl.ninst, lanes, np.int32))
-        b.run_timed()  # warm-up launch
-        ts = [b.run_timed() for _ in range(reps)]
-        times[ci] = np.median(np.stack(ts), axis=0)
+        b.run()
+        torch.cuda.synchronize()
+        times[ci] = b.time_instances(reps)
         if check:
             diff = float((out - ref).abs().max().item())
             if diff > rtol * max(scale, 1e-30):

@@ -9,7 +9,8 @@ dense 8x8 blocks (every row holds its whole block).  Values, x and B are
 f32 U[-1,1] generated on the device.
 
 ``Batch`` wraps the persistent kernel: per-instance schedules (lanes per
-row), a timed launch reporting each instance's time, and hipGraph capture.
+row), launches over the whole batch or a range of instances, per-instance
+event timing for the autotuner, and hipGraph capture.
 """
 import ctypes
 
@@ -165,9 +166,14 @@ class Batch:
     def run(self):
         _lib.check(_lib.lib().taco_hip_batch_run(self._p), "batch run")
 
-    def run_timed(self):
+    def run_range(self, i0, i1):
+        _lib.check(_lib.lib().taco_hip_batch_run_range(self._p, i0, i1), "batch range run")
+
+    def time_instances(self, reps=3):
+        """Per-instance time (ns) of one launch over that instance alone."""
         t = np.zeros(self.wl.ninst)
-        _lib.check(_lib.lib().taco_hip_batch_run_timed(self._p, _p(t)), "batch timed run")
+        _lib.check(_lib.lib().taco_hip_batch_time_instances(self._p, reps, _p(t)),
+                   "batch instance timing")
         return t
 
     def capture(self):

@@ -16,11 +16,10 @@
 //   SpMV: G lanes per row, G in {1, 2, 4, ..., 64} (group-per-row with a
 //         shuffle-tree reduction; G = 1 is thread-per-row)
 //   SpMM: L lanes per row along k (K / L columns per lane), L in {8,16,32,64}
-// Optional per-instance timing: every task adds its duration (the 100 MHz
-// s_memrealtime clock) into the instance's counter, which the autotuner uses
-// to pick the best schedule per instance from one launch per candidate.
-// The queue counter is reset by a one-thread kernel, so the pair (reset,
-// batch) can be captured into a hipGraph and replayed.
+// A launch may cover a contiguous range of instances (their tasks are
+// contiguous), which the autotuner times one instance at a time with HIP
+// events.  The queue counter is reset by a one-thread kernel, so the pair
+// (reset, batch) can be captured into a hipGraph and replayed.
 
 #include <vector>
 
@@ -41,19 +40,13 @@ struct Task {
 
 constexpr int kBatchThreads = 256;
 
-__device__ __forceinline__ uint64_t realtime() {
-  return __builtin_amdgcn_s_memrealtime();
-}
-
 __global__ void k_reset_queue(uint32_t* q) { q[0] = 0; }
 
-template <bool TIMING>
 __global__ __launch_bounds__(kBatchThreads) void k_batch_spmv(
     int32_t ntasks, const Task* __restrict__ tasks, const Inst* __restrict__ insts,
     const int32_t* __restrict__ pos, const int32_t* __restrict__ crd,
     const float* __restrict__ vals, const float* __restrict__ x,
-    float* __restrict__ y, uint32_t* __restrict__ queue,
-    unsigned long long* __restrict__ inst_time) {
+    float* __restrict__ y, uint32_t* __restrict__ queue) {
   __shared__ int32_t s_task;
   const int tid = threadIdx.x;
   for (;;) {
@@ -62,8 +55,6 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmv(
     const int32_t t = s_task;
     __syncthreads();
     if (t >= ntasks) break;  // every thread of the workgroup sees the same t
-    uint64_t t0 = 0;
-    if (TIMING) t0 = realtime();
     const Task tk = tasks[t];
     const Inst in = insts[tk.inst];
     const int G = in.sched;  // lanes per row, power of two <= 64
@@ -82,20 +73,15 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmv(
       for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
       if (lane == 0) iy[r] = acc;
     }
-    if (TIMING) {
-      __syncthreads();
-      if (tid == 0) atomicAdd(inst_time + tk.inst, (unsigned long long)(realtime() - t0));
-    }
   }
 }
 
-template <bool TIMING>
 __global__ __launch_bounds__(kBatchThreads) void k_batch_spmm(
     int32_t ntasks, int32_t K, const Task* __restrict__ tasks,
     const Inst* __restrict__ insts, const int32_t* __restrict__ pos,
     const int32_t* __restrict__ crd, const float* __restrict__ vals,
     const float* __restrict__ B, float* __restrict__ C,
-    uint32_t* __restrict__ queue, unsigned long long* __restrict__ inst_time) {
+    uint32_t* __restrict__ queue) {
   __shared__ int32_t s_task;
   const int tid = threadIdx.x;
   for (;;) {
@@ -104,8 +90,6 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmm(
     const int32_t t = s_task;
     __syncthreads();
     if (t >= ntasks) break;
-    uint64_t t0 = 0;
-    if (TIMING) t0 = realtime();
     const Task tk = tasks[t];
     const Inst in = insts[tk.inst];
     const int L = in.sched;  // lanes per row along k
@@ -133,10 +117,6 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmm(
         const int k = lane + c * L;
         if (k < K) iC[(int64_t)r * K + k] = acc[c];
       }
-    }
-    if (TIMING) {
-      __syncthreads();
-      if (tid == 0) atomicAdd(inst_time + tk.inst, (unsigned long long)(realtime() - t0));
     }
   }
 }
@@ -171,7 +151,6 @@ struct taco_hip_batch {
   Inst* insts = nullptr;
   Task* tasks = nullptr;
   uint32_t* queue = nullptr;
-  unsigned long long* inst_time = nullptr;
   const int32_t* pos = nullptr;
   const int32_t* crd = nullptr;
   const float* vals = nullptr;
@@ -181,6 +160,7 @@ struct taco_hip_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<Inst> host_insts;
+  std::vector<int32_t> task_begin;  // first task of each instance, ninst + 1
 };
 
 namespace {
@@ -188,32 +168,22 @@ void batch_free(taco_hip_batch* b) {
   if (!b) return;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (void* q : {(void*)b->insts, (void*)b->tasks, (void*)b->queue, (void*)b->inst_time})
+  for (void* q : {(void*)b->insts, (void*)b->tasks, (void*)b->queue})
     if (q) (void)hipFree(q);
   delete b;
 }
 
-int launch_batch(taco_hip_batch* b, hipStream_t st, bool timing) {
+int launch_batch(taco_hip_batch* b, hipStream_t st, int32_t t0, int32_t nt) {
+  if (nt <= 0) return 0;
   hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, st, b->queue);
-  if (b->op == 0) {
-    if (timing)
-      hipLaunchKernelGGL(k_batch_spmv<true>, dim3(b->grid), dim3(kBatchThreads), 0, st,
-                         b->ntasks, b->tasks, b->insts, b->pos, b->crd, b->vals, b->in,
-                         b->out, b->queue, b->inst_time);
-    else
-      hipLaunchKernelGGL(k_batch_spmv<false>, dim3(b->grid), dim3(kBatchThreads), 0, st,
-                         b->ntasks, b->tasks, b->insts, b->pos, b->crd, b->vals, b->in,
-                         b->out, b->queue, b->inst_time);
-  } else {
-    if (timing)
-      hipLaunchKernelGGL(k_batch_spmm<true>, dim3(b->grid), dim3(kBatchThreads), 0, st,
-                         b->ntasks, b->K, b->tasks, b->insts, b->pos, b->crd, b->vals,
-                         b->in, b->out, b->queue, b->inst_time);
-    else
-      hipLaunchKernelGGL(k_batch_spmm<false>, dim3(b->grid), dim3(kBatchThreads), 0, st,
-                         b->ntasks, b->K, b->tasks, b->insts, b->pos, b->crd, b->vals,
-                         b->in, b->out, b->queue, b->inst_time);
-  }
+  const unsigned grid = (unsigned)(b->grid < nt ? b->grid : nt);
+  const Task* tasks = b->tasks + t0;
+  if (b->op == 0)
+    hipLaunchKernelGGL(k_batch_spmv, dim3(grid), dim3(kBatchThreads), 0, st, nt, tasks,
+                       b->insts, b->pos, b->crd, b->vals, b->in, b->out, b->queue);
+  else
+    hipLaunchKernelGGL(k_batch_spmm, dim3(grid), dim3(kBatchThreads), 0, st, nt, b->K,
+                       tasks, b->insts, b->pos, b->crd, b->vals, b->in, b->out, b->queue);
   TH_LAUNCH_CHECK("batched kernel");
   return 0;
 }
@@ -266,6 +236,7 @@ int taco_hip_batch_create(int32_t op, int32_t ninst, const int32_t* m,
     in_.n = n[i];
     in_.sched = s;
     in_.pad = 0;
+    b->task_begin.push_back((int32_t)tasks.size());
     for (int32_t r = 0; r < m[i]; r += rows_per_task)
       tasks.push_back(Task{i, r, r + rows_per_task < m[i] ? r + rows_per_task : m[i], 0});
     po += (int64_t)m[i] + 1;
@@ -273,6 +244,7 @@ int taco_hip_batch_create(int32_t op, int32_t ninst, const int32_t* m,
     ro += m[i];
     co += n[i];
   }
+  b->task_begin.push_back((int32_t)tasks.size());
   if (tasks.size() > (size_t)INT32_MAX) {
     batch_free(b);
     return fail("batch: too many tasks");
@@ -281,8 +253,7 @@ int taco_hip_batch_create(int32_t op, int32_t ninst, const int32_t* m,
   hipStream_t st = current_stream();
   if (hipMalloc((void**)&b->insts, sizeof(Inst) * (ninst > 0 ? ninst : 1)) ||
       hipMalloc((void**)&b->tasks, sizeof(Task) * (tasks.empty() ? 1 : tasks.size())) ||
-      hipMalloc((void**)&b->queue, 64) ||
-      hipMalloc((void**)&b->inst_time, sizeof(unsigned long long) * (ninst > 0 ? ninst : 1))) {
+      hipMalloc((void**)&b->queue, 64)) {
     batch_free(b);
     return fail("batch: device allocation failed");
   }
@@ -298,7 +269,6 @@ int taco_hip_batch_create(int32_t op, int32_t ninst, const int32_t* m,
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   b->grid = cus * 8;
-  if (b->grid > b->ntasks) b->grid = b->ntasks > 0 ? b->ntasks : 1;
   *result = b;
   return 0;
 }
@@ -321,26 +291,57 @@ int taco_hip_batch_set_schedules(taco_hip_batch* b, const int32_t* sched) {
 
 int taco_hip_batch_run(taco_hip_batch* b) {
   TH_REQUIRE(b, "batch: null");
-  if (b->ntasks == 0) return 0;
-  if (int rc = launch_batch(b, current_stream(), false)) return rc;
+  if (int rc = launch_batch(b, current_stream(), 0, b->ntasks)) return rc;
   return finish("taco_hip_batch_run");
 }
 
-// One launch with per-instance timing; times[i] receives instance i's summed
-// task durations in nanoseconds (synchronous).
-int taco_hip_batch_run_timed(taco_hip_batch* b, double* times_ns) {
-  TH_REQUIRE(b && times_ns, "batch: null argument");
-  hipStream_t st = current_stream();
+// Runs only instances [i0, i1) (their tasks are contiguous).
+int taco_hip_batch_run_range(taco_hip_batch* b, int32_t i0, int32_t i1) {
+  TH_REQUIRE(b && i0 >= 0 && i0 <= i1 && i1 <= b->ninst, "batch: bad instance range");
+  const int32_t t0 = b->task_begin[(size_t)i0], t1 = b->task_begin[(size_t)i1];
+  if (int rc = launch_batch(b, current_stream(), t0, t1 - t0)) return rc;
+  return finish("taco_hip_batch_run_range");
+}
+
+// Per-instance timing for the autotuner: instance i's tasks are launched
+// `reps` times back to back between two HIP events, every instance in turn
+// on the current stream, with one synchronisation at the end.  times_ns[i]
+// is the mean time of one launch over instance i alone.
+int taco_hip_batch_time_instances(taco_hip_batch* b, int32_t reps, double* times_ns) {
+  TH_REQUIRE(b && times_ns && reps > 0, "batch: bad timing arguments");
   if (b->ninst == 0) return 0;
-  TH_CHECK(hipMemsetAsync(b->inst_time, 0, sizeof(unsigned long long) * b->ninst, st));
-  if (b->ntasks > 0)
-    if (int rc = launch_batch(b, st, true)) return rc;
-  std::vector<unsigned long long> t((size_t)b->ninst);
-  TH_CHECK(hipMemcpyAsync(t.data(), b->inst_time, sizeof(unsigned long long) * b->ninst,
-                          hipMemcpyDeviceToHost, st));
-  TH_CHECK(hipStreamSynchronize(st));
-  for (int32_t i = 0; i < b->ninst; ++i) times_ns[i] = (double)t[(size_t)i] * 10.0;  // 100 MHz
-  return 0;
+  hipStream_t st = current_stream();
+  std::vector<hipEvent_t> ev((size_t)b->ninst + 1, nullptr);
+  int rc = 0;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+      rc = fail("batch: hipEventCreate failed");
+      break;
+    }
+  for (int32_t i = 0; rc == 0 && i < b->ninst; ++i) {
+    if (hipEventRecord(ev[(size_t)i], st) != hipSuccess) {
+      rc = fail("batch: hipEventRecord failed");
+      break;
+    }
+    const int32_t t0 = b->task_begin[(size_t)i], t1 = b->task_begin[(size_t)i + 1];
+    for (int32_t r = 0; rc == 0 && r < reps; ++r) rc = launch_batch(b, st, t0, t1 - t0);
+  }
+  if (rc == 0 && hipEventRecord(ev[(size_t)b->ninst], st) != hipSuccess)
+    rc = fail("batch: hipEventRecord failed");
+  if (rc == 0 && hipEventSynchronize(ev[(size_t)b->ninst]) != hipSuccess)
+    rc = fail("batch: hipEventSynchronize failed");
+  for (int32_t i = 0; rc == 0 && i < b->ninst; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[(size_t)i], ev[(size_t)i + 1]) != hipSuccess) {
+      rc = fail("batch: hipEventElapsedTime failed");
+      break;
+    }
+    times_ns[i] = (double)ms * 1e6 / reps;
+  }
+  for (auto e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return rc;
 }
 
 // Captures (queue reset, batch kernel) into a hipGraph on a private stream.
@@ -353,7 +354,7 @@ int taco_hip_batch_capture(taco_hip_batch* b) {
   if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
     rc = fail("batch: begin capture failed");
   } else {
-    if (b->ntasks > 0) rc = launch_batch(b, cap, false);
+    rc = launch_batch(b, cap, 0, b->ntasks);
     hipGraph_t g = nullptr;
     if (hipStreamEndCapture(cap, &g) != hipSuccess && !rc) rc = fail("batch: end capture failed");
     if (!rc) {

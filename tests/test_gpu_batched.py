@@ -100,6 +100,19 @@ def test_batched_equals_unbatched(wl):
     assert np.array_equal(single.y_of(0), full[:int(wl.m[0])].cpu().numpy().astype(np.float64))
 
 
+def test_run_range_writes_only_its_instances(wl):
+    b = BT.Batch(wl, 0, np.full(wl.ninst, 4, np.int32))
+    wl.y.zero_()
+    b.run_range(5, 9)
+    torch.cuda.synchronize()
+    check_spmv(wl, range(5, 9))
+    r0, r1 = int(wl.row_off[5]), int(wl.row_off[9])
+    assert not wl.y[:r0].any() and not wl.y[r1:].any()
+    b.run_range(9, 9)  # empty range: no launch, no error
+    t = b.time_instances(2)
+    assert t.shape == (wl.ninst,) and np.all(np.isfinite(t)) and np.all(t > 0)
+
+
 def test_autotune_batch(wl):
     best, times, cands, rejected = autotune.tune_batch(wl, 0, reps=2)
     assert rejected == []
