@@ -174,6 +174,13 @@ int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* plan, taco_tensor_t* y,
 void taco_hip_spmv_plan_destroy(taco_hip_spmv_plan* plan);
 /* info[0..4] = parts, slabs, slab width, row weight, max rows in a part. */
 int taco_hip_spmv_plan_info(const taco_hip_spmv_plan* plan, int64_t* info);
+/* Kernel timing: HIP events bracket the main kernel (k_slab_spmv) of the
+ * next `launches` compute calls on their stream (0 stops and releases).
+ * plan_kernel_time waits for the last recorded call and returns the summed
+ * kernel time and the number of recorded calls. */
+int taco_hip_spmv_plan_time_kernel(taco_hip_spmv_plan* plan, int32_t launches);
+int taco_hip_spmv_plan_kernel_time(taco_hip_spmv_plan* plan, double* total_ms,
+                                   int64_t* launches);
 
 /* ======================================================================
  * Batched uneven SpMV / SpMM (f32): many CSR instances in one persistent

@@ -89,6 +89,8 @@ _SIGS = {
     "taco_hip_spmv_plan_compute": (c_int, [c_void_p, P_T, P_T, c_int32]),
     "taco_hip_spmv_plan_destroy": (None, [c_void_p]),
     "taco_hip_spmv_plan_info": (c_int, [c_void_p, POINTER(c_int64)]),
+    "taco_hip_spmv_plan_time_kernel": (c_int, [c_void_p, c_int32]),
+    "taco_hip_spmv_plan_kernel_time": (c_int, [c_void_p, POINTER(c_double), POINTER(c_int64)]),
     "taco_hip_batch_create": (c_int, [c_int32, c_int32, c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_int32, c_int32, c_void_p, c_void_p,
                                       c_void_p, c_void_p, c_void_p, POINTER(c_void_p)]),
@@ -223,6 +225,16 @@ class SpmvPlan:
         check(lib().taco_hip_spmv_plan_info(self._p, out))
         return dict(zip(("parts", "slabs", "slab_width", "row_weight",
                          "max_local_rows"), list(out)))
+
+    def time_kernel(self, launches):
+        """Bracket the main kernel of the next `launches` computes with events."""
+        check(lib().taco_hip_spmv_plan_time_kernel(self._p, launches), "plan timing")
+
+    def kernel_time(self):
+        """(mean main-kernel ms, recorded launches)."""
+        ms, n = c_double(), c_int64()
+        check(lib().taco_hip_spmv_plan_kernel_time(self._p, ctypes.byref(ms), ctypes.byref(n)))
+        return (ms.value / n.value if n.value else float("nan")), n.value
 
     def compute(self, y_desc, x_desc, unroll=0):
         check(lib().taco_hip_spmv_plan_compute(self._p, y_desc.ptr, x_desc.ptr,

@@ -28,6 +28,7 @@
 // to run.  The MERGE_PATH schedule is the deterministic one.
 
 #include <climits>
+#include <vector>
 
 #include "common.h"
 #include "device.h"
@@ -254,11 +255,23 @@ struct taco_hip_spmv_plan {
   int32_t* carry_row = nullptr;   // parts
   double* carry_val = nullptr;    // parts
   int32_t max_local_rows = 0;
+  // Optional kernel timing: event pairs around k_slab_spmv of the next
+  // `timing_cap` compute calls.
+  std::vector<hipEvent_t> timing_ev;
+  int32_t timing_cap = 0, timing_count = 0;
 };
 
 namespace {
+void timing_release(taco_hip_spmv_plan* p) {
+  for (hipEvent_t e : p->timing_ev)
+    if (e) (void)hipEventDestroy(e);
+  p->timing_ev.clear();
+  p->timing_cap = p->timing_count = 0;
+}
+
 void plan_free(taco_hip_spmv_plan* p) {
   if (!p) return;
+  timing_release(p);
   for (void* q : {(void*)p->part_row, (void*)p->part_entry, (void*)p->off,
                   (void*)p->ent, (void*)p->val2, (void*)p->carry_row,
                   (void*)p->carry_val})
@@ -374,19 +387,50 @@ extern "C" int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* p,
         p->m, p->slabs, p->slab_width, p->part_row, p->off, p->ent,         \
         p->val2, xp, yp, p->carry_row, p->carry_val);                       \
     break;
-  switch (unroll <= 0 ? 2 : unroll) {
-    TH_SLAB(1) TH_SLAB(2) TH_SLAB(4) TH_SLAB(8)
-    default:
-      return fail("spmv plan: unroll %d unsupported", unroll);
-  }
+  const int u = unroll <= 0 ? 2 : unroll;
+  if (u != 1 && u != 2 && u != 4 && u != 8) return fail("spmv plan: unroll %d unsupported", unroll);
+  const bool timed = p->timing_count < p->timing_cap;
+  if (timed) TH_CHECK(hipEventRecord(p->timing_ev[2 * (size_t)p->timing_count], st));
+  switch (u) { TH_SLAB(1) TH_SLAB(2) TH_SLAB(4) TH_SLAB(8) }
 #undef TH_SLAB
   TH_LAUNCH_CHECK("k_slab_spmv");
+  if (timed) TH_CHECK(hipEventRecord(p->timing_ev[2 * (size_t)p->timing_count++ + 1], st));
   k_slab_fixup<<<(p->parts + 255) / 256, 256, 0, st>>>(p->parts, p->m, p->carry_row,
                                                        p->carry_val, yp);
   return finish("taco_hip_spmv_plan_compute");
 }
 
 extern "C" void taco_hip_spmv_plan_destroy(taco_hip_spmv_plan* p) { plan_free(p); }
+
+extern "C" int taco_hip_spmv_plan_time_kernel(taco_hip_spmv_plan* p, int32_t launches) {
+  TH_REQUIRE(p && launches >= 0, "spmv plan: bad timing arguments");
+  timing_release(p);
+  p->timing_ev.assign(2 * (size_t)launches, nullptr);
+  for (hipEvent_t& e : p->timing_ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+      timing_release(p);
+      return fail("spmv plan: hipEventCreate failed");
+    }
+  p->timing_cap = launches;
+  return 0;
+}
+
+extern "C" int taco_hip_spmv_plan_kernel_time(taco_hip_spmv_plan* p, double* total_ms,
+                                              int64_t* launches) {
+  TH_REQUIRE(p && total_ms && launches, "spmv plan: null argument");
+  double sum = 0.0;
+  if (p->timing_count > 0)
+    TH_CHECK(hipEventSynchronize(p->timing_ev[2 * (size_t)p->timing_count - 1]));
+  for (int32_t i = 0; i < p->timing_count; ++i) {
+    float ms = 0.f;
+    TH_CHECK(hipEventElapsedTime(&ms, p->timing_ev[2 * (size_t)i], p->timing_ev[2 * (size_t)i + 1]));
+    sum += ms;
+  }
+  *total_ms = sum;
+  *launches = p->timing_count;
+  return 0;
+}
 
 extern "C" int taco_hip_spmv_plan_info(const taco_hip_spmv_plan* p, int64_t* info) {
   TH_REQUIRE(p && info, "spmv plan: null argument");
