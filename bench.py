@@ -227,7 +227,8 @@ def bench_spmv(args, A, x, name, workload_desc):
         "dtype": "f64", "data": "synthetic (seed-generated)",
         "config": {"workload": name, "desc": workload_desc, "m": A.m, "n": A.n,
                    "nnz": A.nnz, "schedule": f"slab-tiled plan parts={plan.info()['parts']} "
-                                             f"slab_width={plan.info()['slab_width']} unroll={args.unroll}",
+                                             f"slab_width={plan.info()['slab_width']} unroll={args.unroll or 1} "
+                                             f"threads=256 wave_reduce=1",
                    "parallelism": "single GPU"},
         "pct_of_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS,
         "gflops": 2.0 * A.nnz / (ms * 1e-3) / 1e9,
@@ -417,7 +418,7 @@ def main():
     ap.add_argument("--workload", default="spmv_powerlaw")
     ap.add_argument("--items", type=int, default=8, help="SpMV merge items per thread")
     ap.add_argument("--parts", type=int, default=1024, help="slab plan parts")
-    ap.add_argument("--unroll", type=int, default=2, help="slab plan entries per thread per step")
+    ap.add_argument("--unroll", type=int, default=0, help="slab plan entries per thread per step (0 = library default)")
     ap.add_argument("--spmm-items", type=int, default=32)
     ap.add_argument("--mfma", type=int, default=1)
     ap.add_argument("--cpu-reps", type=int, default=30)

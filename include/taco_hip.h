@@ -174,6 +174,11 @@ int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* plan, taco_tensor_t* y,
 void taco_hip_spmv_plan_destroy(taco_hip_spmv_plan* plan);
 /* info[0..4] = parts, slabs, slab width, row weight, max rows in a part. */
 int taco_hip_spmv_plan_info(const taco_hip_spmv_plan* plan, int64_t* info);
+/* Launch form of the main kernel: workgroup size 256 (default) or 512, and
+ * whether runs of one row are summed across the wave before the LDS
+ * atomics (1, default) or every entry adds its own product (0). */
+int taco_hip_spmv_plan_configure(taco_hip_spmv_plan* plan, int32_t threads,
+                                 int32_t wave_reduce);
 /* Kernel timing: HIP events bracket the main kernel (k_slab_spmv) of the
  * next `launches` compute calls on their stream (0 stops and releases).
  * plan_kernel_time waits for the last recorded call and returns the summed
@@ -231,6 +236,48 @@ int taco_hip_device_exclusive_scan(const int32_t* counts, int32_t m,
  * the next request of the same slot on this device; never freed by the
  * caller.  NULL on failure. */
 void* taco_hip_workspace_get(size_t bytes, int slot);
+
+/* ======================================================================
+ * Learned schedule selector.  No reference counterpart (taco schedules are
+ * written by hand).  Features of a CSR matrix and an MLP
+ * 32 -> 128 -> 128 -> 25 (24 schedule logits: 0..12 SpMV, 13..23 SpMM, and
+ * a predicted log time) with forward, backward and Adam kernels in FP32.
+ * ==================================================================== */
+/* Exact integer statistics of A (30 values): m, n, nnz, then sum of row
+ * lengths, sum of squared lengths, max length, empty rows, 16 log2-length
+ * bins, entries in rows >= 64 and >= 1024, then over min(m, 4096) evenly
+ * spaced rows: rows, entries, entries within 64 of the diagonal column,
+ * entries in the 8x8 diagonal block, and the summed distance from the
+ * diagonal in 1/4096 units.  Synchronous. */
+int taco_hip_csr_stats(taco_tensor_t* A, int64_t* stats);
+/* The 32 selector features of an operation (op 0 SpMV, 1 SpMM of width K)
+ * from the statistics above (host arithmetic only). */
+int taco_hip_features_from_stats(const int64_t* stats, int32_t op, int32_t K,
+                                 float* features);
+int taco_hip_features(taco_tensor_t* A, int32_t op, int32_t K, float* features);
+
+typedef struct taco_hip_mlp taco_hip_mlp;
+/* Parameters: one flat f32 array W1[128][32], b1, W2[128][128], b2,
+ * W3[25][128], b3 (23,961 values). */
+int64_t taco_hip_mlp_param_count(void);
+int taco_hip_mlp_create(taco_hip_mlp** out);
+void taco_hip_mlp_destroy(taco_hip_mlp* net);
+/* Host parameter copies; set also clears the optimizer state. */
+int taco_hip_mlp_set_params(taco_hip_mlp* net, const float* host);
+int taco_hip_mlp_get_params(taco_hip_mlp* net, float* host);
+/* Z[B][25] = network(X[B][32]) (device pointers, current stream). */
+int taco_hip_mlp_forward(taco_hip_mlp* net, const float* X, int32_t B, float* Z);
+/* Loss (mean over the batch) and its gradient for device inputs X[B][32],
+ * target classes cls[B] and log times logt[B]; gradients copied to
+ * grads_host (23,961 floats) when non-null.  Synchronous. */
+int taco_hip_mlp_gradients(taco_hip_mlp* net, const float* X, const int32_t* cls,
+                           const float* logt, int32_t B, float* grads_host,
+                           float* loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8; an L2 term weight_decay * p
+ * is added to the gradient); loss optional (synchronous when requested). */
+int taco_hip_mlp_train_step(taco_hip_mlp* net, const float* X, const int32_t* cls,
+                            const float* logt, int32_t B, float lr,
+                            float weight_decay, float* loss);
 
 /* ======================================================================
  * Code generation (CodeGen_HIP) and the JIT module loader.

@@ -7,8 +7,8 @@ point raises, so a run can never silently compute on the CPU.
 """
 import ctypes
 import os
-from ctypes import (POINTER, Structure, c_char_p, c_double, c_int, c_int32,
-                    c_int64, c_size_t, c_uint8, c_uint64, c_void_p)
+from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int,
+                    c_int32, c_int64, c_size_t, c_uint8, c_uint64, c_void_p)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtaco_hip.so")
@@ -90,6 +90,7 @@ _SIGS = {
     "taco_hip_spmv_plan_destroy": (None, [c_void_p]),
     "taco_hip_spmv_plan_info": (c_int, [c_void_p, POINTER(c_int64)]),
     "taco_hip_spmv_plan_time_kernel": (c_int, [c_void_p, c_int32]),
+    "taco_hip_spmv_plan_configure": (c_int, [c_void_p, c_int32, c_int32]),
     "taco_hip_spmv_plan_kernel_time": (c_int, [c_void_p, POINTER(c_double), POINTER(c_int64)]),
     "taco_hip_batch_create": (c_int, [c_int32, c_int32, c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_int32, c_int32, c_void_p, c_void_p,
@@ -105,6 +106,19 @@ _SIGS = {
     "taco_hip_gen_uniform_f32_device": (c_int, [c_int64, c_uint64, c_uint64, c_double,
                                                 c_double, c_void_p]),
     "taco_hip_device_exclusive_scan": (c_int, [c_void_p, c_int32, c_void_p]),
+    "taco_hip_csr_stats": (c_int, [P_T, POINTER(c_int64)]),
+    "taco_hip_features_from_stats": (c_int, [c_void_p, c_int32, c_int32, c_void_p]),
+    "taco_hip_features": (c_int, [P_T, c_int32, c_int32, c_void_p]),
+    "taco_hip_mlp_param_count": (c_int64, []),
+    "taco_hip_mlp_create": (c_int, [POINTER(c_void_p)]),
+    "taco_hip_mlp_destroy": (None, [c_void_p]),
+    "taco_hip_mlp_set_params": (c_int, [c_void_p, c_void_p]),
+    "taco_hip_mlp_get_params": (c_int, [c_void_p, c_void_p]),
+    "taco_hip_mlp_forward": (c_int, [c_void_p, c_void_p, c_int32, c_void_p]),
+    "taco_hip_mlp_gradients": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+                                       c_void_p, POINTER(c_float)]),
+    "taco_hip_mlp_train_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+                                        c_float, c_float, POINTER(c_float)]),
     "taco_hip_workspace_get": (c_void_p, [c_size_t, c_int]),
     "taco_hip_generate": (c_void_p, [c_void_p, c_int32]),
     "taco_hip_free_string": (None, [c_void_p]),
@@ -225,6 +239,9 @@ class SpmvPlan:
         check(lib().taco_hip_spmv_plan_info(self._p, out))
         return dict(zip(("parts", "slabs", "slab_width", "row_weight",
                          "max_local_rows"), list(out)))
+
+    def configure(self, threads=0, wave_reduce=1):
+        check(lib().taco_hip_spmv_plan_configure(self._p, threads, wave_reduce), "plan configure")
 
     def time_kernel(self, launches):
         """Bracket the main kernel of the next `launches` computes with events."""

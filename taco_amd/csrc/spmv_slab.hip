@@ -20,7 +20,19 @@
 // merge-path kernel.
 //
 // Packed entry: (local row << 18) | (column - slab * slab_width), so an
-// entry stays 4 + 8 bytes.
+// entry stays 4 + 8 bytes.  Each (part, slab) segment starts at a multiple
+// of kAlign entries (the gaps hold zeroed filler that is never summed).
+//
+// Two forms of the main kernel accumulate the row partials in LDS:
+//   k_slab_spmv_wave (default): a wave takes blocks of 64 consecutive
+//     entries; a segmented wave sum by row (DPP row shifts and broadcasts)
+//     leaves one total per row run, added by one lane — so a long row costs
+//     one LDS atomic per 64 entries instead of 64 atomics to one address;
+//   k_slab_spmv: a thread takes U consecutive entries (vector loads), sums
+//     the runs of one row among them and adds each run with an LDS atomic.
+// On the 2M-row power-law matrix both take about the same time: the kernel
+// is bound by the x gathers that miss L2 and by the streaming of A, not by
+// the LDS atomics.
 //
 // Row partial sums accumulate with LDS float atomics, so the order of the
 // additions inside a row is not fixed: results agree with the sequential sum
@@ -40,6 +52,9 @@ namespace {
 constexpr int kColBits = 18;
 constexpr uint32_t kColMask = (1u << kColBits) - 1;
 constexpr int kPlanThreads = 256;
+constexpr int kDefaultUnroll = 1;
+// Segment start alignment (entries); the largest per-thread chunk.
+constexpr int kAlign = 8;
 // Row partials live in dynamic LDS sized by the plan's largest part; this
 // caps one part at 76 KB of f64 partials.
 constexpr int kMaxLocalRows = 9728;
@@ -77,11 +92,11 @@ __global__ __launch_bounds__(256) void k_slab_partition(
   part_entry[p] = (int32_t)z;
 }
 
-// Per part: entries per slab, then offsets off[p*(S+1) + s].
-__global__ __launch_bounds__(256) void k_slab_offsets(
+// Per part: entries per slab (into cnt) and the part's padded size.
+__global__ __launch_bounds__(256) void k_slab_counts(
     int32_t parts, int32_t slabs, int32_t slab_width,
     const int32_t* __restrict__ crd, const int32_t* __restrict__ part_entry,
-    int32_t* __restrict__ off) {
+    int32_t* __restrict__ cnt, int32_t* __restrict__ part_size) {
   __shared__ int32_t s_count[1024];
   const int32_t p = blockIdx.x;
   for (int s = threadIdx.x; s < slabs; s += blockDim.x) s_count[s] = 0;
@@ -91,21 +106,34 @@ __global__ __launch_bounds__(256) void k_slab_offsets(
     atomicAdd(&s_count[crd[z] / slab_width], 1);
   __syncthreads();
   if (threadIdx.x == 0) {
-    int32_t acc = z0;
-    int32_t* o = off + (int64_t)p * (slabs + 1);
+    int64_t acc = 0;
     for (int s = 0; s < slabs; ++s) {
-      o[s] = acc;
-      acc += s_count[s];
+      cnt[(int64_t)p * slabs + s] = s_count[s];
+      acc += (s_count[s] + kAlign - 1) / kAlign * kAlign;
     }
-    o[slabs] = acc;
+    part_size[p] = acc > INT32_MAX ? INT32_MAX : (int32_t)acc;
   }
 }
 
-// Per part: stable scatter of the part's entries into slab order, packing
-// the local row into the column word.  Entries are taken in chunks of the
-// workgroup size; the rank of an entry inside its slab is the number of
-// earlier entries of the same slab in earlier chunks (running cursor) plus
-// in earlier lanes of this chunk (ballots per wave, prefix over waves).
+// Per part: aligned segment starts off[p][0..slabs] from the part's base
+// (exclusive scan of the padded part sizes) and segment ends seg_end[p][s].
+__global__ __launch_bounds__(256) void k_slab_layout(
+    int32_t parts, int32_t slabs, const int32_t* __restrict__ part_base,
+    const int32_t* __restrict__ cnt, int32_t* __restrict__ off,
+    int32_t* __restrict__ seg_end) {
+  const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= parts) return;
+  int32_t at = part_base[p];
+  int32_t* o = off + (int64_t)p * (slabs + 1);
+  for (int s = 0; s < slabs; ++s) {
+    const int32_t c = cnt[(int64_t)p * slabs + s];
+    o[s] = at;
+    seg_end[(int64_t)p * slabs + s] = at + c;
+    at += (c + kAlign - 1) / kAlign * kAlign;
+  }
+  o[slabs] = at;
+}
+
 __global__ __launch_bounds__(256) void k_slab_scatter(
     int32_t slabs, int32_t slab_width, const int32_t* __restrict__ pos,
     const int32_t* __restrict__ crd, const double* __restrict__ vals,
@@ -166,10 +194,51 @@ __global__ __launch_bounds__(256) void k_slab_scatter(
   }
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// Loads U consecutive packed entries and values starting at an index that is
+// a multiple of U (16-byte vector loads for U >= 2).
 template <int U>
-__global__ __launch_bounds__(kPlanThreads) void k_slab_spmv(
+__device__ __forceinline__ void load_chunk(const uint32_t* __restrict__ ent,
+                                           const double* __restrict__ val2,
+                                           int64_t e, uint32_t (&pk)[U], double (&v)[U]) {
+  if constexpr (U == 1) {
+    pk[0] = stream_load(ent + e);
+    v[0] = stream_load(val2 + e);
+  } else if constexpr (U == 2) {
+    const f64x2 a = __builtin_nontemporal_load((const f64x2*)(val2 + e));
+    pk[0] = stream_load(ent + e);
+    pk[1] = stream_load(ent + e + 1);
+    v[0] = a.x;
+    v[1] = a.y;
+  } else {
+#pragma unroll
+    for (int q = 0; q < U / 4; ++q) {
+      const u32x4 a = __builtin_nontemporal_load((const u32x4*)(ent + e) + q);
+      pk[4 * q + 0] = a.x;
+      pk[4 * q + 1] = a.y;
+      pk[4 * q + 2] = a.z;
+      pk[4 * q + 3] = a.w;
+    }
+#pragma unroll
+    for (int q = 0; q < U / 2; ++q) {
+      const f64x2 a = __builtin_nontemporal_load((const f64x2*)(val2 + e) + q);
+      v[2 * q + 0] = a.x;
+      v[2 * q + 1] = a.y;
+    }
+  }
+}
+
+// One workgroup per part.  Each thread takes U consecutive entries of the
+// current slab segment; entries of one row are adjacent in a segment, so
+// the thread sums each run of equal rows in a register and adds the run to
+// the row's LDS partial with one atomic.
+template <int U, int T>
+__global__ __launch_bounds__(T) void k_slab_spmv(
     int32_t m, int32_t slabs, int32_t slab_width,
     const int32_t* __restrict__ part_row, const int32_t* __restrict__ off,
+    const int32_t* __restrict__ seg_end,
     const uint32_t* __restrict__ ent, const double* __restrict__ val2,
     const double* __restrict__ x, double* __restrict__ y,
     int32_t* __restrict__ carry_row, double* __restrict__ carry_val) {
@@ -178,23 +247,115 @@ __global__ __launch_bounds__(kPlanThreads) void k_slab_spmv(
   const int tid = threadIdx.x;
   const int32_t r0 = part_row[p], r1 = part_row[p + 1];
   const int32_t nrows = r1 - r0 + 1;  // local rows, <= kMaxLocalRows by plan
-  for (int32_t q = tid; q < nrows; q += kPlanThreads) s_part[q] = 0;
+  for (int32_t q = tid; q < nrows; q += T) s_part[q] = 0;
   __syncthreads();
   const int32_t* o = off + (int64_t)p * (slabs + 1);
+  const int32_t* se = seg_end + (int64_t)p * slabs;
   for (int32_t s = 0; s < slabs; ++s) {
-    const int32_t e0 = o[s], e1 = o[s + 1];
+    const int32_t e0 = o[s], e1 = se[s];
+    const int32_t nchunks = (e1 - e0 + U - 1) / U;
     const double* xs = x + (int64_t)s * slab_width;
-    // Branch-free loads: out-of-range lanes re-read the chunk's last entry
-    // (a valid address) and drop their contribution, so the compiler issues
-    // all U streaming loads and then all U gathers back to back instead of
-    // waiting on each conditional load in turn.
-    for (int32_t base = e0; base < e1; base += kPlanThreads * U) {
+    for (int32_t c0 = 0; c0 < nchunks; c0 += T) {
+      const int32_t c = c0 + tid;
+      // Threads past the last chunk re-read it (a valid, aligned address)
+      // and sum nothing, so every thread issues the same loads.
+      const int32_t e = e0 + (c < nchunks ? c : nchunks - 1) * U;
+      uint32_t pk[U];
+      double v[U];
+      load_chunk<U>(ent, val2, e, pk, v);
+      double xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) xv[u] = xs[pk[u] & kColMask];
+      const int32_t live = c < nchunks ? e1 - e : 0;  // entries of this thread
+      double acc = 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (u < live) {
+          acc = fma(v[u], xv[u], acc);
+          const uint32_t row = pk[u] >> kColBits;
+          const bool last = u == U - 1 || u + 1 >= live || (pk[u + 1] >> kColBits) != row;
+          if (last) {
+            atomicAdd(&s_part[row], acc);
+            acc = 0.0;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // Rows r0 .. r1-1 complete here; row r1 is carried.
+  for (int32_t q = tid; q < nrows - 1; q += T) y[r0 + q] = s_part[q];
+  if (tid == 0) {
+    carry_row[p] = r1;
+    carry_val[p] = s_part[nrows - 1];
+  }
+}
+
+// f64 DPP move: lanes whose source is invalid or outside row_mask read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+
+// Segmented inclusive sum over the wave: lane l gets the sum of lanes
+// ss .. l, where ss <= l is the first lane of l's segment.  Row shifts by
+// 1, 2, 4, 8 within each 16-lane row, then the row broadcasts of lanes 15
+// (into rows 1 and 3), 47 (into row 3) and 31 (into rows 2 and 3).
+__device__ __forceinline__ double wave_segmented_sum(double v, int lane, int ss) {
+  double t;
+  t = dpp_f64<0x111, 0xf>(v);  // row_shr:1
+  if (lane - 1 >= ss) v += t;
+  t = dpp_f64<0x112, 0xf>(v);  // row_shr:2
+  if (lane - 2 >= ss) v += t;
+  t = dpp_f64<0x114, 0xf>(v);  // row_shr:4
+  if (lane - 4 >= ss) v += t;
+  t = dpp_f64<0x118, 0xf>(v);  // row_shr:8
+  if (lane - 8 >= ss) v += t;
+  t = dpp_f64<0x142, 0xa>(v);  // row_bcast:15 -> rows 1, 3 read lanes 15, 47
+  if ((lane & 16) && (lane & ~15) - 1 >= ss) v += t;
+  t = dpp_f64<0x143, 0xc>(v);  // row_bcast:31 -> rows 2, 3 read lane 31
+  if ((lane & 32) && 31 >= ss) v += t;
+  return v;
+}
+
+// Wave-reduced variant: each wave takes U blocks of 64 consecutive entries
+// per step (lane l holds entry l of a block).  Entries of one row are
+// adjacent, so a segmented wave sum by row leaves each row's block total in
+// its last lane, which alone adds it to the LDS partial: a row spanning
+// the whole block costs one LDS atomic instead of 64 to one address.
+template <int U, int T>
+__global__ __launch_bounds__(T) void k_slab_spmv_wave(
+    int32_t m, int32_t slabs, int32_t slab_width,
+    const int32_t* __restrict__ part_row, const int32_t* __restrict__ off,
+    const int32_t* __restrict__ seg_end,
+    const uint32_t* __restrict__ ent, const double* __restrict__ val2,
+    const double* __restrict__ x, double* __restrict__ y,
+    int32_t* __restrict__ carry_row, double* __restrict__ carry_val) {
+  extern __shared__ double s_part[];
+  constexpr int kWaves = T / kWave;
+  const int32_t p = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid % kWave, wave = tid / kWave;
+  const int32_t r0 = part_row[p], r1 = part_row[p + 1];
+  const int32_t nrows = r1 - r0 + 1;  // local rows, <= kMaxLocalRows by plan
+  for (int32_t q = tid; q < nrows; q += T) s_part[q] = 0;
+  __syncthreads();
+  const int32_t* o = off + (int64_t)p * (slabs + 1);
+  const int32_t* se = seg_end + (int64_t)p * slabs;
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  for (int32_t s = 0; s < slabs; ++s) {
+    const int32_t e0 = o[s], e1 = se[s];
+    const double* xs = x + (int64_t)s * slab_width;
+    for (int32_t base = e0; base < e1; base += T * U) {
       uint32_t pk[U];
       double v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int32_t e = base + tid + u * kPlanThreads;
-        const int32_t ec = e < e1 ? e : e1 - 1;
+        const int32_t e = base + (u * kWaves + wave) * kWave + lane;
+        const int32_t ec = e < e1 ? e : e1 - 1;  // valid address; summed as 0
         pk[u] = stream_load(ent + ec);
         v[u] = stream_load(val2 + ec);
       }
@@ -203,14 +364,23 @@ __global__ __launch_bounds__(kPlanThreads) void k_slab_spmv(
       for (int u = 0; u < U; ++u) xv[u] = xs[pk[u] & kColMask];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int32_t e = base + tid + u * kPlanThreads;
-        if (e < e1) atomicAdd(&s_part[pk[u] >> kColBits], v[u] * xv[u]);
+        const int32_t e = base + (u * kWaves + wave) * kWave + lane;
+        const bool live = e < e1;
+        const uint32_t row = pk[u] >> kColBits;
+        const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(
+            -1, (int)row, 0x138, 0xf, 0xf, false);  // wave_shr:1, lane 0 reads -1
+        const bool head = !live || lane == 0 || prev != row;
+        const uint64_t heads = __ballot(head);
+        const int ss = 63 - __clzll(heads & upto);
+        const double sum = wave_segmented_sum(live ? v[u] * xv[u] : 0.0, lane, ss);
+        const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1);
+        if (live && tail) atomicAdd(&s_part[row], sum);
       }
     }
   }
   __syncthreads();
   // Rows r0 .. r1-1 complete here; row r1 is carried.
-  for (int32_t q = tid; q < nrows - 1; q += kPlanThreads) y[r0 + q] = s_part[q];
+  for (int32_t q = tid; q < nrows - 1; q += T) y[r0 + q] = s_part[q];
   if (tid == 0) {
     carry_row[p] = r1;
     carry_val[p] = s_part[nrows - 1];
@@ -249,12 +419,16 @@ struct taco_hip_spmv_plan {
   int64_t row_weight = 0;
   int32_t* part_row = nullptr;    // parts + 1
   int32_t* part_entry = nullptr;  // parts + 1
-  int32_t* off = nullptr;         // parts * (slabs + 1)
-  uint32_t* ent = nullptr;        // nnz
-  double* val2 = nullptr;         // nnz
+  int32_t* off = nullptr;         // parts * (slabs + 1), aligned segment starts
+  int32_t* seg_end = nullptr;     // parts * slabs, segment ends
+  int64_t padded = 0;             // entries in the padded layout
+  uint32_t* ent = nullptr;        // padded
+  double* val2 = nullptr;         // padded
   int32_t* carry_row = nullptr;   // parts
   double* carry_val = nullptr;    // parts
   int32_t max_local_rows = 0;
+  int32_t threads = kPlanThreads;  // workgroup size of the main kernel
+  int32_t wave_reduce = 1;         // k_slab_spmv_wave (1) or k_slab_spmv (0)
   // Optional kernel timing: event pairs around k_slab_spmv of the next
   // `timing_cap` compute calls.
   std::vector<hipEvent_t> timing_ev;
@@ -273,8 +447,8 @@ void plan_free(taco_hip_spmv_plan* p) {
   if (!p) return;
   timing_release(p);
   for (void* q : {(void*)p->part_row, (void*)p->part_entry, (void*)p->off,
-                  (void*)p->ent, (void*)p->val2, (void*)p->carry_row,
-                  (void*)p->carry_val})
+                  (void*)p->seg_end, (void*)p->ent, (void*)p->val2,
+                  (void*)p->carry_row, (void*)p->carry_val})
     if (q) (void)hipFree(q);
   delete p;
 }
@@ -322,8 +496,7 @@ extern "C" int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
   if (alloc((void**)&p->part_row, sizeof(int32_t) * (parts + 1)) ||
       alloc((void**)&p->part_entry, sizeof(int32_t) * (parts + 1)) ||
       alloc((void**)&p->off, sizeof(int32_t) * (size_t)parts * (slabs + 1)) ||
-      alloc((void**)&p->ent, sizeof(uint32_t) * (size_t)a.nnz) ||
-      alloc((void**)&p->val2, sizeof(double) * (size_t)a.nnz) ||
+      alloc((void**)&p->seg_end, sizeof(int32_t) * (size_t)parts * slabs) ||
       alloc((void**)&p->carry_row, sizeof(int32_t) * parts) ||
       alloc((void**)&p->carry_val, sizeof(double) * parts)) {
     plan_free(p);
@@ -349,14 +522,56 @@ extern "C" int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
     return fail("spmv plan: a part holds %d rows (max %d)", host_max, kMaxLocalRows);
   }
   p->max_local_rows = host_max;
-  if (a.nnz > 0) {
-    k_slab_offsets<<<parts, 256, 0, st>>>(parts, slabs, slab_width, a.crd,
-                                         p->part_entry, p->off);
+  // Layout: per-slab counts, padded part sizes, their scan, aligned starts.
+  int32_t* cnt = nullptr;
+  int32_t* part_size = nullptr;
+  int32_t* part_base = nullptr;
+  auto release_tmp = [&]() {
+    for (void* q : {(void*)cnt, (void*)part_size, (void*)part_base})
+      if (q) (void)hipFree(q);
+  };
+  if (alloc((void**)&cnt, sizeof(int32_t) * (size_t)parts * slabs) ||
+      alloc((void**)&part_size, sizeof(int32_t) * parts) ||
+      alloc((void**)&part_base, sizeof(int32_t) * (parts + 1))) {
+    release_tmp();
+    plan_free(p);
+    return fail("spmv plan: device allocation failed");
+  }
+  k_slab_counts<<<parts, 256, 0, st>>>(parts, slabs, slab_width, a.crd, p->part_entry, cnt,
+                                        part_size);
+  if (int rc = taco_hip_device_exclusive_scan(part_size, parts, part_base)) {
+    release_tmp();
+    plan_free(p);
+    return rc;
+  }
+  int32_t total = 0;
+  if (hipMemcpyAsync(&total, part_base + parts, sizeof(int32_t), hipMemcpyDeviceToHost, st) ||
+      hipStreamSynchronize(st)) {
+    release_tmp();
+    plan_free(p);
+    return fail("spmv plan: layout scan failed");
+  }
+  p->padded = total;
+  if (alloc((void**)&p->ent, sizeof(uint32_t) * (size_t)total) ||
+      alloc((void**)&p->val2, sizeof(double) * (size_t)total)) {
+    release_tmp();
+    plan_free(p);
+    return fail("spmv plan: device allocation failed");
+  }
+  // Filler entries (row 0, column 0, value 0) are loaded but never summed.
+  (void)hipMemsetAsync(p->ent, 0, sizeof(uint32_t) * (size_t)total, st);
+  (void)hipMemsetAsync(p->val2, 0, sizeof(double) * (size_t)total, st);
+  k_slab_layout<<<(parts + 255) / 256, 256, 0, st>>>(parts, slabs, part_base, cnt, p->off,
+                                                     p->seg_end);
+  if (a.nnz > 0)
     k_slab_scatter<<<parts, 256, 0, st>>>(slabs, slab_width, a.pos, a.crd,
                                          (const double*)a.vals, p->part_row,
                                          p->part_entry, p->off, p->ent, p->val2);
-  } else {
-    (void)hipMemsetAsync(p->off, 0, sizeof(int32_t) * (size_t)parts * (slabs + 1), st);
+  const hipError_t build_err = hipStreamSynchronize(st);
+  release_tmp();
+  if (build_err != hipSuccess) {
+    plan_free(p);
+    return fail("spmv plan: build kernels failed");
   }
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
     plan_free(p);
@@ -381,18 +596,30 @@ extern "C" int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* p,
   const double* xp = (const double*)xv.vals;
   double* yp = (double*)yv.vals;
   const size_t lds = sizeof(double) * (size_t)(p->max_local_rows > 0 ? p->max_local_rows : 1);
-#define TH_SLAB(U)                                                          \
-  case U:                                                                   \
-    k_slab_spmv<U><<<p->parts, kPlanThreads, lds, st>>>(                    \
-        p->m, p->slabs, p->slab_width, p->part_row, p->off, p->ent,         \
-        p->val2, xp, yp, p->carry_row, p->carry_val);                       \
-    break;
-  const int u = unroll <= 0 ? 2 : unroll;
+  const int u = unroll <= 0 ? kDefaultUnroll : unroll;
+  const int th = p->threads;
   if (u != 1 && u != 2 && u != 4 && u != 8) return fail("spmv plan: unroll %d unsupported", unroll);
   const bool timed = p->timing_count < p->timing_cap;
   if (timed) TH_CHECK(hipEventRecord(p->timing_ev[2 * (size_t)p->timing_count], st));
-  switch (u) { TH_SLAB(1) TH_SLAB(2) TH_SLAB(4) TH_SLAB(8) }
+#define TH_SLAB(U, T)                                                           \
+  if (u == U && th == T)                                                        \
+    k_slab_spmv<U, T><<<p->parts, T, lds, st>>>(                                \
+        p->m, p->slabs, p->slab_width, p->part_row, p->off, p->seg_end, p->ent, \
+        p->val2, xp, yp, p->carry_row, p->carry_val);
+#define TH_SLAB_W(U, T)                                                         \
+  if (u == U && th == T)                                                        \
+    k_slab_spmv_wave<U, T><<<p->parts, T, lds, st>>>(                           \
+        p->m, p->slabs, p->slab_width, p->part_row, p->off, p->seg_end, p->ent, \
+        p->val2, xp, yp, p->carry_row, p->carry_val);
+  if (p->wave_reduce) {
+    TH_SLAB_W(1, 256) TH_SLAB_W(2, 256) TH_SLAB_W(4, 256) TH_SLAB_W(8, 256)
+    TH_SLAB_W(1, 512) TH_SLAB_W(2, 512) TH_SLAB_W(4, 512) TH_SLAB_W(8, 512)
+  } else {
+    TH_SLAB(1, 256) TH_SLAB(2, 256) TH_SLAB(4, 256) TH_SLAB(8, 256)
+    TH_SLAB(1, 512) TH_SLAB(2, 512) TH_SLAB(4, 512) TH_SLAB(8, 512)
+  }
 #undef TH_SLAB
+#undef TH_SLAB_W
   TH_LAUNCH_CHECK("k_slab_spmv");
   if (timed) TH_CHECK(hipEventRecord(p->timing_ev[2 * (size_t)p->timing_count++ + 1], st));
   k_slab_fixup<<<(p->parts + 255) / 256, 256, 0, st>>>(p->parts, p->m, p->carry_row,
@@ -401,6 +628,16 @@ extern "C" int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* p,
 }
 
 extern "C" void taco_hip_spmv_plan_destroy(taco_hip_spmv_plan* p) { plan_free(p); }
+
+extern "C" int taco_hip_spmv_plan_configure(taco_hip_spmv_plan* p, int32_t threads,
+                                           int32_t wave_reduce) {
+  TH_REQUIRE(p, "spmv plan: null plan");
+  if (threads <= 0) threads = kPlanThreads;
+  TH_REQUIRE(threads == 256 || threads == 512, "spmv plan: threads must be 256 or 512");
+  p->threads = threads;
+  p->wave_reduce = wave_reduce != 0;
+  return 0;
+}
 
 extern "C" int taco_hip_spmv_plan_time_kernel(taco_hip_spmv_plan* p, int32_t launches) {
   TH_REQUIRE(p && launches >= 0, "spmv plan: bad timing arguments");

@@ -1,0 +1,135 @@
+"""Builds the selector's timing database and trains it on the GPU.
+
+    python -m taco_amd.train_selector timings --seed 7 --start 0 --count 2000 --out DIR
+    python -m taco_amd.train_selector train --db DIR/timings_*.npz --out taco_amd/data/selector.npz
+    python -m taco_amd.train_selector evaluate --db DIR/heldout_*.npz --weights ...
+
+`timings` generates problems (taco_amd.selector.make_problem(seed, i) for i
+in [start, start+count)), computes their features on the GPU and times every
+schedule class with the exhaustive autotuner, writing one npz per call
+(features F[N, 32], times T[N, 24] in ms, op, K, rows, nnz, structure).
+`train` fits the MLP with the library's HIP training kernels.  `evaluate`
+reports learned vs exhaustive vs default totals on a database.
+"""
+import argparse
+import glob
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cmd_timings(args):
+    import torch
+    from . import _lib, selector as S
+    torch.cuda.set_device(0)
+    _lib.check(_lib.lib().taco_hip_set_stream(torch.cuda.current_stream().cuda_stream))
+    n = args.count
+    F = np.zeros((n, S.N_FEATURES), np.float32)
+    T = np.full((n, S.N_CLASSES), np.inf)
+    meta = {k: np.zeros(n, np.int64) for k in ("op", "K", "m", "nnz", "structure")}
+    t0 = time.time()
+    for j in range(n):
+        i = args.start + j
+        A, op, K, d = S.make_problem(args.seed, i, rmin=args.rmin, rmax=args.rmax)
+        prob = S.DeviceProblem(A, op, K, seed=i)
+        F[j] = S.features(prob.a, op, K)
+        T[j] = S.time_classes(prob, reps=args.reps)
+        meta["op"][j], meta["K"][j], meta["m"][j], meta["nnz"][j] = op, K, A.m, A.nnz
+        meta["structure"][j] = S.STRUCTURES.index(d["structure"])
+        del prob, A
+        if (j + 1) % args.progress == 0 or j + 1 == n:
+            log(f"[{j + 1}/{n}] {time.time() - t0:.0f}s; last: {d} best "
+                f"{S.CLASS_NAMES[int(np.argmin(T[j]))]} {np.min(T[j]):.3f} ms")
+    out = f"{args.out}/timings_s{args.seed}_{args.start}_{args.start + n}.npz"
+    np.savez(out, F=F, T=T, seed=args.seed, start=args.start, rmin=args.rmin, rmax=args.rmax,
+             **meta)
+    log(f"wrote {out} in {time.time() - t0:.0f}s")
+
+
+def load_db(patterns):
+    parts = []
+    for pat in patterns:
+        for f in sorted(glob.glob(pat)):
+            parts.append(np.load(f, allow_pickle=False))
+    if not parts:
+        raise SystemExit(f"no timing files match {patterns}")
+    cat = {k: np.concatenate([p[k] for p in parts]) for k in ("F", "T", "op", "K", "m", "nnz",
+                                                               "structure")}
+    ok = np.isfinite(cat["T"].min(axis=1))
+    return {k: v[ok] for k, v in cat.items()}
+
+
+def cmd_train(args):
+    import torch
+    from . import _lib, selector as S
+    torch.cuda.set_device(0)
+    _lib.check(_lib.lib().taco_hip_set_stream(torch.cuda.current_stream().cuda_stream))
+    db = load_db(args.db)
+    log(f"training on {len(db['F'])} problems")
+    t0 = time.time()
+    params, mu, sigma, hist = S.train(db["F"], db["T"], epochs=args.epochs, batch=args.batch,
+                                      lr=args.lr, weight_decay=args.wd, seed=args.seed, log=log)
+    secs = time.time() - t0
+    sel_choice = None
+    S.save(args.out, params, mu, sigma,
+           {"problems": int(len(db["F"])), "epochs": args.epochs, "batch": args.batch,
+            "lr": args.lr, "train_seconds": secs})
+    sel = S.Selector(args.out)
+    sel_choice, _ = sel.choose_from_features(db["F"])
+    ev = S.evaluate(sel_choice, db["T"], db["op"])
+    log(f"trained in {secs:.1f}s; on the training set: {json.dumps(ev)}")
+    print(json.dumps({"train_seconds": secs, "final_loss": hist[-1][1], "train_eval": ev}))
+
+
+def cmd_evaluate(args):
+    import torch
+    from . import _lib, selector as S
+    torch.cuda.set_device(0)
+    _lib.check(_lib.lib().taco_hip_set_stream(torch.cuda.current_stream().cuda_stream))
+    db = load_db(args.db)
+    sel = S.Selector(args.weights)
+    choice, _ = sel.choose_from_features(db["F"])
+    out = {"problems": int(len(db["F"]))}
+    out.update(S.evaluate(choice, db["T"], db["op"]))
+    for op in (0, 1):
+        k = db["op"] == op
+        if k.any():
+            out[("spmv", "spmm")[op]] = S.evaluate(choice[k], db["T"][k], db["op"][k])
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("timings")
+    t.add_argument("--seed", type=int, default=7)
+    t.add_argument("--start", type=int, default=0)
+    t.add_argument("--count", type=int, default=100)
+    t.add_argument("--rmin", type=int, default=10_000)
+    t.add_argument("--rmax", type=int, default=2_000_000)
+    t.add_argument("--reps", type=int, default=3)
+    t.add_argument("--progress", type=int, default=50)
+    t.add_argument("--out", default="gpurun_out")
+    r = sub.add_parser("train")
+    r.add_argument("--db", nargs="+", required=True)
+    r.add_argument("--out", required=True)
+    r.add_argument("--epochs", type=int, default=300)
+    r.add_argument("--batch", type=int, default=256)
+    r.add_argument("--lr", type=float, default=2e-3)
+    r.add_argument("--wd", type=float, default=1e-5)
+    r.add_argument("--seed", type=int, default=0)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--db", nargs="+", required=True)
+    e.add_argument("--weights", required=True)
+    args = ap.parse_args()
+    {"timings": cmd_timings, "train": cmd_train, "evaluate": cmd_evaluate}[args.cmd](args)
+
+
+if __name__ == "__main__":
+    main()

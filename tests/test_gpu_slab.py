@@ -11,9 +11,10 @@ from tests import gpu_util as G
 pytestmark = pytest.mark.gpu
 
 
-def run(A, x, parts=0, slab=0, unroll=0):
+def run(A, x, parts=0, slab=0, unroll=0, threads=0, wave_reduce=1):
     a = G.csr_desc(A)
     plan = _lib.SpmvPlan(a, parts, slab)
+    plan.configure(threads, wave_reduce)
     y = torch.full((A.m,), float("nan"), dtype=torch.float64, device="cuda")
     xd = G.dev(x)
     plan.compute(D.dense(y), D.dense(xd), unroll)
@@ -31,16 +32,35 @@ CASES = [
 ]
 
 
+FORMS = [(1, 256, 1), (2, 256, 1), (4, 512, 1), (8, 256, 1), (1, 256, 0), (4, 256, 0),
+         (8, 512, 0)]
+
+
 @pytest.mark.parametrize("name,make,parts,slab", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("unroll", [1, 4, 8])
-def test_slab_spmv(name, make, parts, slab, unroll):
+@pytest.mark.parametrize("unroll,threads,wave_reduce", FORMS,
+                         ids=[f"u{u}t{t}w{w}" for u, t, w in FORMS])
+def test_slab_spmv(name, make, parts, slab, unroll, threads, wave_reduce):
     A = make()
-    print(f"slab spmv {name} m={A.m} nnz={A.nnz} parts={parts} slab={slab} unroll={unroll}", flush=True)
+    print(f"slab spmv {name} m={A.m} nnz={A.nnz} parts={parts} slab={slab} unroll={unroll} "
+          f"threads={threads} wave_reduce={wave_reduce}", flush=True)
     x = W.uniform(A.n, 5, W.STREAM_X)
-    y, info = run(A, x, parts, slab, unroll)
+    y, info = run(A, x, parts, slab, unroll, threads, wave_reduce)
     ye, cond = oracle.exact_spmv(A, x)
     rep = oracle.error_report(y, ye, cond, 1e-12)
     assert rep["ok"], (rep, info)
+
+
+def test_slab_block_boundaries():
+    """Rows whose runs start and end at every lane position of a 64-entry
+    block and across the 16-lane rows of the wave (run lengths 1..70)."""
+    counts = np.concatenate([np.arange(1, 71), np.arange(70, 0, -1), [0, 0, 129, 1, 63, 65]])
+    A = W.csr_from_counts(len(counts), 5000, counts.astype(np.int32), 47)
+    x = W.uniform(A.n, 6, W.STREAM_X)
+    ye, cond = oracle.exact_spmv(A, x)
+    for unroll, threads in [(1, 256), (2, 512)]:
+        y, info = run(A, x, 4, 1024, unroll, threads, 1)
+        rep = oracle.error_report(y, ye, cond, 1e-12)
+        assert rep["ok"], (unroll, threads, rep, info)
 
 
 def test_slab_plan_rejects_f32():
