@@ -153,6 +153,11 @@ __global__ __launch_bounds__(256) void k_loss(int32_t B, const float* __restrict
   const float* z = Z + (int64_t)b * kOut;
   float* dz = dZ + (int64_t)b * kOut;
   const int32_t c = cls[b];
+  if (c < 0 || c >= kClasses) {  // not a class: no gradient, NaN loss
+    for (int j = 0; j < kOut; ++j) dz[j] = 0.f;
+    loss[b] = NAN;
+    return;
+  }
   const int lo = c < kSpmvClasses ? 0 : kSpmvClasses;
   const int hi = c < kSpmvClasses ? kSpmvClasses : kClasses;
   float mx = -INFINITY;
