@@ -217,7 +217,7 @@ def bench_spmv(args, A, x, name, workload_desc):
                "kind": kind, "ms": cms, "median_ms": float(np.median(cpu_ts)),
                "sample": f"whole workload, {len(cpu_ts)} timed compute() calls "
                          f"(10% trimmed mean), JIT excluded, inputs in host DRAM"}
-    traffic = profiled_traffic("k_slab_spmv")
+    traffic = profiled_traffic("k_slab_spmv_wave")
     kernel_gbs = nbytes / (kernel_ms * 1e-3) / 1e9
     return {
         "metric": "SpMV HBM GB/s (% of 8 TB/s), SpMM GFLOP/s: learned vs default schedule at tol",
@@ -392,6 +392,120 @@ def bench_spgemm(args):
     }
 
 
+def bench_batched(args, op):
+    """4,096 uneven CSR instances (rows log-uniform in [1e3, 2e5], row
+    lengths uniform in [2, 64], mixed uniform / banded / 8x8-block
+    structure, f32) in one persistent hipGraph-replayed launch: taco's
+    default (one lane per row for SpMV, a 32-lane group per row for SpMM)
+    against the exhaustive per-instance autotune."""
+    torch, L, _lib = gpu_setup()
+    from taco_amd import autotune, batch as BT
+    t0 = time.perf_counter()
+    wl = BT.BatchedWorkload(ninst=args.instances, rmin=1000, rmax=200000, kmin=2, kmax=64,
+                            seed=4, width=32, spmm=(op == 1))
+    gen_s = time.perf_counter() - t0
+    log(f"batched: {wl.ninst} instances, {wl.total_rows} rows, {wl.total_nnz} entries, "
+        f"generated in {gen_s:.1f}s")
+    t0 = time.perf_counter()
+    best, times, cands, rejected = autotune.tune_batch(wl, op, reps=3)
+    tune_s = time.perf_counter() - t0
+    timer = Timer(torch)
+    res = {}
+    for name, sched in (("default", BT.default_schedule(op, wl.ninst)), ("autotuned", best)):
+        b = BT.Batch(wl, op, sched)
+        b.capture()
+        ms, wall = timer.run(b.launch_graph, args.steps, args.warmup)
+        res[name] = {"ms": ms, "wall_ms": wall, "info": b.info()}
+        del b
+    nbytes = wl.algorithmic_bytes(op)
+    flops = 2.0 * wl.total_nnz * (1 if op == 0 else wl.width)
+    ms = res["autotuned"]["ms"]
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    lanes = BT.SPMV_LANES if op == 0 else BT.SPMM_LANES
+    return {
+        "metric": f"batched {'SpMV' if op == 0 else 'SpMM'} GB/s (one persistent graph launch)",
+        "value": gbs, "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seed-generated)",
+        "config": {"workload": f"batched_{'spmv' if op == 0 else 'spmm_k32'}_4096",
+                   "instances": wl.ninst, "rows": wl.total_rows, "nnz": wl.total_nnz,
+                   "schedule": "per-instance lanes per row from the exhaustive autotuner"},
+        "gflops": flops / (ms * 1e-3) / 1e9, "nnz_per_s": wl.total_nnz / (ms * 1e-3),
+        "default_schedule": {"ms": res["default"]["ms"],
+                             "speedup_vs_default": res["default"]["ms"] / ms},
+        "autotune": {"seconds": tune_s, "rejected": rejected,
+                     "chosen_lanes_histogram": {int(l): int((best == l).sum()) for l in lanes}},
+        "generate_seconds": gen_s,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_batch_spmv" if op == 0 else "k_batch_spmm",
+                     "algorithmic_bytes_per_launch": nbytes},
+        "cpu_baseline": None,
+    }
+
+
+def bench_selector(args):
+    """The learned schedule selector on held-out problems (seed 8, never
+    trained on): each problem is regenerated, every class re-timed on this
+    GPU (the exhaustive autotune), and the selector's pick compared with the
+    exhaustive best and with taco's default schedule (thread per row).
+    Also times one decision (features + network) on the 2M-row power-law
+    matrix against the SpMV it chooses."""
+    torch, L, _lib = gpu_setup()
+    from taco_amd import selector as S, descriptors as D, workloads as W
+    sel = S.Selector()
+    F, T, ops = [], [], []
+    t0 = time.perf_counter()
+    for i in range(args.problems):
+        A, op, K, d = S.make_problem(8, i)
+        prob = S.DeviceProblem(A, op, K, seed=i)
+        F.append(S.features(prob.a, op, K))
+        T.append(S.time_classes(prob, reps=5))
+        ops.append(op)
+        del prob, A
+        if (i + 1) % 25 == 0:
+            log(f"selector: {i + 1}/{args.problems} problems timed")
+    F, T, ops = np.array(F), np.array(T), np.array(ops)
+    tune_s = time.perf_counter() - t0
+    choice, _ = sel.choose_from_features(F)
+    ev = S.evaluate(choice, T, ops)
+    per_op = {}
+    for o, name in ((0, "spmv"), (1, "spmm")):
+        k = ops == o
+        if k.any():
+            per_op[name] = S.evaluate(choice[k], T[k], ops[k])
+    # decision latency on the power-law matrix
+    A = W.powerlaw_2m()
+    a = D.csr(torch.from_numpy(A.pos).cuda(), torch.from_numpy(A.crd).cuda(),
+              torch.from_numpy(A.vals).cuda(), A.shape)
+    lat = []
+    for _ in range(args.warmup + 20):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        c, sched_c, pred_ms = sel.choose(a, 0)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    lat = float(np.median(lat[args.warmup:]))
+    x = D.dense(torch.from_numpy(W.uniform(A.n, 2, W.STREAM_X)).cuda())
+    y = D.dense(torch.empty(A.m, dtype=torch.float64, device="cuda"))
+    kms, _ = Timer(torch).run(lambda: _lib.check(L.taco_hip_spmv_compute(y.ptr, a.ptr, x.ptr, sched_c)),
+                              20, 3)
+    return {
+        "metric": "learned-schedule speedup over the default schedule (held-out problems)",
+        "value": ev["default_over_learned"], "unit": "x", "n_gpus": 1, "steps": args.problems,
+        "warmup": args.warmup, "ms_per_step": ev["learned_ms"] / len(F),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 network; f64 SpMV, f32 SpMM", "data": "synthetic (seed-generated)",
+        "config": {"workload": "selector_heldout", "problems": int(len(F)),
+                   "rows": "log-uniform 10k-2M", "structures": list(S.STRUCTURES),
+                   "spmm_widths": list(S.SPMM_WIDTHS)},
+        "learned_over_exhaustive": ev["learned_over_exhaustive"],
+        "evaluation": ev, "per_op": per_op, "retime_seconds": tune_s,
+        "decision": {"latency_ms": lat, "class": S.CLASS_NAMES[c], "predicted_ms": pred_ms,
+                     "measured_kernel_ms": kms, "matrix": "powerlaw_2m"},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def copy_calibration(args):
     """A plain device copy of known size (2 GiB read + 2 GiB written per
     launch), run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE into
@@ -423,6 +537,8 @@ def main():
     ap.add_argument("--mfma", type=int, default=1)
     ap.add_argument("--cpu-reps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--instances", type=int, default=4096, help="batched workloads")
+    ap.add_argument("--problems", type=int, default=200, help="selector held-out problems")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -448,6 +564,10 @@ def main():
         out = bench_spgemm(args)
     elif w == "copy_calibration":
         out = copy_calibration(args)
+    elif w == "selector":
+        out = bench_selector(args)
+    elif w in ("batched_spmv", "batched_spmm"):
+        out = bench_batched(args, 0 if w == "batched_spmv" else 1)
     else:
         sys.exit(f"unknown workload {w}")
     print(json.dumps(out), flush=True)

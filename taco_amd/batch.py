@@ -193,7 +193,8 @@ class Batch:
             self._p = None
 
 
-SPMV_LANES = (1, 2, 4, 8, 16, 32, 64)
+# Per-instance SpMV schedules: lanes per row, or 0 for the stream schedule.
+SPMV_LANES = (1, 2, 4, 8, 16, 32, 64, 0)
 SPMM_LANES = (8, 16, 32, 64)
 
 

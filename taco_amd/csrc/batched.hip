@@ -14,7 +14,10 @@
 // a persistent grid (a few workgroups per CU) pulls tasks from a queue
 // counter.  Each instance carries its own schedule:
 //   SpMV: G lanes per row, G in {1, 2, 4, ..., 64} (group-per-row with a
-//         shuffle-tree reduction; G = 1 is thread-per-row)
+//         shuffle-tree reduction; G = 1 is thread-per-row), or G = 0, the
+//         stream schedule: the workgroup's products go through LDS in
+//         coalesced chunks and each thread sums one row from LDS (balanced
+//         by entries instead of rows; suits short rows)
 //   SpMM: L lanes per row along k (K / L columns per lane), L in {8,16,32,64}
 // A launch may cover a contiguous range of instances (their tasks are
 // contiguous), which the autotuner times one instance at a time with HIP
@@ -39,6 +42,8 @@ struct Task {
 };
 
 constexpr int kBatchThreads = 256;
+// Products staged in LDS per chunk by the stream schedule (16 KB).
+constexpr int kStreamChunk = 4096;
 
 __global__ void k_reset_queue(uint32_t* q) { q[0] = 0; }
 
@@ -48,6 +53,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmv(
     const float* __restrict__ vals, const float* __restrict__ x,
     float* __restrict__ y, uint32_t* __restrict__ queue) {
   __shared__ int32_t s_task;
+  __shared__ float s_prod[kStreamChunk];
   const int tid = threadIdx.x;
   for (;;) {
     if (tid == 0) s_task = (int32_t)atomicAdd(queue, 1u);
@@ -57,14 +63,38 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmv(
     if (t >= ntasks) break;  // every thread of the workgroup sees the same t
     const Task tk = tasks[t];
     const Inst in = insts[tk.inst];
-    const int G = in.sched;  // lanes per row, power of two <= 64
-    const int lane = tid & (G - 1);
-    const int rows_per_pass = kBatchThreads / G;
+    const int G = in.sched;  // lanes per row (power of two <= 64), 0 = stream
     const int32_t* ip = pos + in.pos_off;
     const int32_t* ic = crd + in.ent_off;
     const float* iv = vals + in.ent_off;
     const float* ix = x + in.col_off;
     float* iy = y + in.row_off;
+    if (G == 0) {
+      // Stream: the workgroup computes the products of up to kStreamChunk
+      // consecutive entries into LDS (coalesced loads, one entry per
+      // thread), then each thread adds up its own row's products.  Passes
+      // of kBatchThreads rows; rows longer than a chunk span chunks.
+      for (int32_t rb = tk.r0; rb < tk.r1; rb += kBatchThreads) {
+        const int32_t re_ = rb + kBatchThreads < tk.r1 ? rb + kBatchThreads : tk.r1;
+        const int32_t r = rb + tid;
+        const int32_t q0 = r < re_ ? ip[r] : 0, q1 = r < re_ ? ip[r + 1] : 0;
+        const int32_t e_begin = ip[rb], e_end = ip[re_];
+        float acc = 0.f;
+        for (int32_t c0 = e_begin; c0 < e_end; c0 += kStreamChunk) {
+          const int32_t c1 = c0 + kStreamChunk < e_end ? c0 + kStreamChunk : e_end;
+          for (int32_t q = c0 + tid; q < c1; q += kBatchThreads)
+            s_prod[q - c0] = stream_load(iv + q) * ix[stream_load(ic + q)];
+          __syncthreads();
+          const int32_t a = q0 > c0 ? q0 : c0, b = q1 < c1 ? q1 : c1;
+          for (int32_t q = a; q < b; ++q) acc += s_prod[q - c0];
+          __syncthreads();
+        }
+        if (r < re_) iy[r] = acc;
+      }
+      continue;
+    }
+    const int lane = tid & (G - 1);
+    const int rows_per_pass = kBatchThreads / G;
     for (int32_t r = tk.r0 + tid / G; r < tk.r1; r += rows_per_pass) {
       float acc = 0.f;
       const int32_t end = ip[r + 1];
@@ -189,6 +219,11 @@ int launch_batch(taco_hip_batch* b, hipStream_t st, int32_t t0, int32_t nt) {
 }
 
 bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+// SpMV: 0 (stream) or lanes per row 1..64; SpMM: lanes 8..64 with K <= 4*lanes.
+bool valid_sched(int op, int s, int K) {
+  if (op == 0) return s == 0 || (pow2(s) && s <= 64);
+  return pow2(s) && s >= 8 && s <= 64 && K <= 4 * s;
+}
 }  // namespace
 
 extern "C" {
@@ -219,7 +254,7 @@ int taco_hip_batch_create(int32_t op, int32_t ninst, const int32_t* m,
   int64_t po = 0, eo = 0, ro = 0, co = 0;
   for (int32_t i = 0; i < ninst; ++i) {
     const int s = sched[i];
-    if (!pow2(s) || s > 64 || (op == 1 && (s < 8 || K > 4 * s))) {
+    if (!valid_sched(op, s, K)) {
       batch_free(b);
       return fail("batch: instance %d has schedule %d, not a valid lane count", i, s);
     }
@@ -278,7 +313,7 @@ int taco_hip_batch_set_schedules(taco_hip_batch* b, const int32_t* sched) {
   TH_REQUIRE(b && sched, "batch: null argument");
   for (int32_t i = 0; i < b->ninst; ++i) {
     const int s = sched[i];
-    TH_REQUIRE(pow2(s) && s <= 64 && (b->op == 0 || (s >= 8 && b->K <= 4 * s)),
+    TH_REQUIRE(valid_sched(b->op, s, b->K),
                "batch: instance %d has schedule %d, not a valid lane count", i, s);
     b->host_insts[(size_t)i].sched = s;
   }
