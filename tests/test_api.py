@@ -1,0 +1,99 @@
+"""The taco-API facade's host logic: statements and schedules recognised as
+the reference writes them, and taco-style errors for the rest (no GPU)."""
+import numpy as np
+import pytest
+
+from taco_amd import _lib
+from taco_amd import api as taco
+from tests import schedules as S
+
+
+def spmv_setup(dtype=taco.Float64):
+    i, j = taco.IndexVar("i"), taco.IndexVar("j")
+    A = taco.Tensor("A", [50, 40], taco.CSR, dtype)
+    x = taco.Tensor("x", [40], taco.Format([taco.Dense]), dtype)
+    y = taco.Tensor("y", [50], taco.Format([taco.Dense]), dtype)
+    pre = A[i, j] * x[j]
+    y[i] = pre
+    return A, x, y, i, j, pre
+
+
+def recognise(y, stmt):
+    kind, a, b = taco._statement_kind(stmt.assignment)
+    return kind, taco._recognise(stmt, kind, a)
+
+
+def test_default_schedule_is_taco_default():
+    A, x, y, i, j, pre = spmv_setup()
+    assert recognise(y, y.get_assignment().concretize()) == ("spmv", (_lib.GEN_DEFAULT, 256, 0, 0, 0, 0))
+
+
+def test_reference_spmv_schedules():
+    A, x, y, i, j, pre = spmv_setup()
+    st = y.get_assignment().concretize()
+    assert recognise(y, S.schedule_spmv_gpu(st, A, pre, i, j)) == \
+        ("spmv", (_lib.GEN_NNZ_SPLIT, 256, 8, 32, 0, 0))
+    assert recognise(y, S.schedule_spmv_gpu(st, A, pre, i, j, 4, 512)) == \
+        ("spmv", (_lib.GEN_NNZ_SPLIT, 512, 4, 32, 0, 0))
+    assert recognise(y, S.schedule_spmv_rows_gpu(st, A, pre, i, j)) == \
+        ("spmv", (_lib.GEN_WARP_ROWS, 256, 0, 32, 1, 0))
+    assert recognise(y, S.schedule_spmv_rows_gpu(st, A, pre, i, j, 2, 128)) == \
+        ("spmv", (_lib.GEN_WARP_ROWS, 128, 0, 32, 2, 0))
+    assert recognise(y, S.schedule_spmv_thread_per_row_gpu(st, A, pre, i, j, 128)) == \
+        ("spmv", (_lib.GEN_DEFAULT, 128, 0, 0, 0, 0))
+
+
+def test_reference_spmm_schedule():
+    i, j, k = taco.IndexVar("i"), taco.IndexVar("j"), taco.IndexVar("k")
+    A = taco.Tensor("A", [30, 20], taco.CSR)
+    B = taco.Tensor("B", [20, 64], taco.dense_matrix)
+    C = taco.Tensor("C", [30, 64], taco.dense_matrix)
+    pre = A[i, j] * B[j, k]
+    C[i, k] = pre
+    st = S.schedule_spmm_gpu(C.get_assignment().concretize(), A, pre, i, j, k)
+    assert recognise(C, st) == ("spmm", (_lib.GEN_NNZ_SPLIT, 256, 8, 32, 0, 4))
+
+
+def test_spgemm_recognised():
+    i, j, k = taco.IndexVar("i"), taco.IndexVar("j"), taco.IndexVar("k")
+    A = taco.Tensor("A", [30, 20], taco.CSR)
+    B = taco.Tensor("B", [20, 25], taco.CSR)
+    C = taco.Tensor("C", [30, 25], taco.CSR)
+    C[i, k] = A[i, j] * B[j, k]
+    assert recognise(C, C.get_assignment().concretize())[0] == "spgemm"
+
+
+def test_unsupported_schedule_and_statement_raise():
+    A, x, y, i, j, pre = spmv_setup()
+    st = y.get_assignment().concretize()
+    o, n = taco.IndexVar("o"), taco.IndexVar("n")
+    bad = st.split(j, o, n, 4).parallelize(o, taco.ParallelUnit.GPUBlock, taco.OutputRaceStrategy.NoRaces)
+    with pytest.raises(taco.TacoError, match="schedule not supported"):
+        recognise(y, bad)
+    # rows schedule whose thread loop asks for atomics instead of a temporary
+    wrong = S.schedule_spmv_rows_gpu(st, A, pre, i, j)
+    wrong.commands[-1] = ("parallelize", wrong.commands[-1][1], taco.ParallelUnit.GPUThread,
+                          taco.OutputRaceStrategy.Atomics)
+    with pytest.raises(taco.TacoError):
+        recognise(y, wrong)
+    z = taco.Tensor("z", [40], taco.Format([taco.Dense]))
+    w = taco.Tensor("w", [50], taco.Format([taco.Dense]))
+    w[i] = x[j] * z[j]
+    with pytest.raises(taco.TacoError, match="CSR"):
+        taco._statement_kind(w.get_assignment())
+
+
+def test_pack_and_state_errors():
+    A = taco.Tensor("A", [3, 4], taco.CSR)
+    A.insert([2, 1], 5.0)
+    A.insert([0, 3], 1.0)
+    A.insert([2, 0], 2.0)
+    A.pack()
+    pos, crd, vals = A.to_csr()
+    assert pos.tolist() == [0, 1, 1, 3] and crd.tolist() == [3, 0, 1]
+    assert vals.tolist() == [1.0, 2.0, 5.0]
+    y = taco.Tensor("y", [3])
+    with pytest.raises(taco.TacoError, match="compile"):
+        y.compute()
+    with pytest.raises(taco.TacoError, match="format"):
+        taco.Tensor("T", [2, 2, 2], taco.Format([taco.Dense, taco.Sparse, taco.Sparse]))

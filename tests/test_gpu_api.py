@@ -1,0 +1,112 @@
+"""The taco-API facade end to end, written like the reference's GPU tests
+(test/tests-scheduling-eval.cpp:1210-1307): build tensors with insert/pack,
+schedule with the reference's commands, compile/assemble/compute, and
+compare with taco's results on the same inputs (the golden vectors the real
+taco wrote) and with the default schedule."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from taco_amd import api as taco
+from taco_amd import workloads as W
+from tests import schedules as S
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def csr_tensor(name, dims, pos, crd, vals, insert=True):
+    A = taco.Tensor(name, dims, taco.CSR)
+    if insert:  # the reference's way: insert every entry, then pack
+        for r in range(dims[0]):
+            for q in range(pos[r], pos[r + 1]):
+                A.insert([r, int(crd[q])], float(vals[q]))
+        A.pack()
+    else:
+        A.set_csr(pos, crd, vals)
+    return A
+
+
+@pytest.mark.parametrize("schedule", ["default", "spmv_gpu", "rows", "thread_per_row"])
+def test_spmv_gpu_reference_inputs(schedule):
+    d = np.load(os.path.join(GOLD, "ref_tests.npz"))
+    i, j = taco.IndexVar("i"), taco.IndexVar("j")
+    A = csr_tensor("A", [102, 103], d["spmvgpu_pos"], d["spmvgpu_crd"], d["spmvgpu_vals"])
+    x = taco.Tensor("x", [103], taco.Format([taco.Dense]))
+    for q, v in enumerate(d["spmvgpu_x"]):
+        x.insert([q], float(v))
+    x.pack()
+    y = taco.Tensor("y", [102], taco.Format([taco.Dense]))
+    precomputed = A[i, j] * x[j]
+    y[i] = precomputed
+    stmt = y.get_assignment().concretize()
+    if schedule == "spmv_gpu":
+        stmt = S.schedule_spmv_gpu(stmt, A, precomputed, i, j)
+    elif schedule == "rows":
+        stmt = S.schedule_spmv_rows_gpu(stmt, A, precomputed, i, j)
+    elif schedule == "thread_per_row":
+        stmt = S.schedule_spmv_thread_per_row_gpu(stmt, A, precomputed, i, j)
+    y.compile(stmt)
+    y.assemble()
+    y.compute()
+    expected = taco.Tensor("expected", [102], taco.Format([taco.Dense]))
+    expected[i] = A[i, j] * x[j]
+    expected.evaluate()
+    assert np.array_equal(y.to_numpy(), expected.to_numpy())
+    assert np.array_equal(y.to_numpy(), d["spmvgpu_y"])
+
+
+def test_spmm_gpu_reference_inputs():
+    d = np.load(os.path.join(GOLD, "ref_tests.npz"))
+    i, j, k = taco.IndexVar("i"), taco.IndexVar("j"), taco.IndexVar("k")
+    A = csr_tensor("A", [102, 103], d["spmmgpu_pos"], d["spmmgpu_crd"], d["spmmgpu_vals"])
+    B = taco.Tensor("B", [103, 128], taco.dense_matrix)
+    B.set_values(d["spmmgpu_B"])
+    C = taco.Tensor("C", [102, 128], taco.dense_matrix)
+    precomputed = A[i, j] * B[j, k]
+    C[i, k] = precomputed
+    stmt = S.schedule_spmm_gpu(C.get_assignment().concretize(), A, precomputed, i, j, k)
+    C.compile(stmt)
+    C.assemble()
+    C.compute()
+    assert np.array_equal(C.to_numpy(), d["spmmgpu_C"])
+
+
+def test_spgemm_two_phase_matches_oracle():
+    i, j, k = taco.IndexVar("i"), taco.IndexVar("j"), taco.IndexVar("k")
+    Ah = W.csr_fixed(400, 300, 6, 51)
+    Bh = W.csr_powerlaw(300, 350, seed=52, kmax=40)
+    A = csr_tensor("A", [400, 300], Ah.pos, Ah.crd, Ah.vals, insert=False)
+    B = csr_tensor("B", [300, 350], Bh.pos, Bh.crd, Bh.vals, insert=False)
+    C = taco.Tensor("C", [400, 350], taco.CSR)
+    C[i, k] = A[i, j] * B[j, k]
+    C.compile()
+    C.assemble()
+    C.compute()
+    pos, crd, vals = C.to_csr()
+    rpos, rcrd = oracle.spgemm_assemble(Ah, Bh)
+    assert np.array_equal(pos, rpos) and np.array_equal(crd, rcrd)
+    ex, cond = oracle.exact_spgemm_values(Ah, Bh, rpos, rcrd)
+    assert oracle.error_report(vals, ex, cond, 1e-12)["ok"]
+
+
+def test_f32_spmv_through_the_api():
+    Ah = W.csr_powerlaw(3000, 2500, seed=53, kmax=500)
+    xh = W.uniform(2500, 53, W.STREAM_X)
+    i, j = taco.IndexVar("i"), taco.IndexVar("j")
+    A = taco.Tensor("A", [3000, 2500], taco.CSR, taco.Float32)
+    A.set_csr(Ah.pos, Ah.crd, Ah.vals)
+    x = taco.Tensor("x", [2500], taco.Format([taco.Dense]), taco.Float32)
+    x.set_values(xh)
+    y = taco.Tensor("y", [3000], taco.Format([taco.Dense]), taco.Float32)
+    pre = A[i, j] * x[j]
+    y[i] = pre
+    y.compile(S.schedule_spmv_gpu(y.get_assignment().concretize(), A, pre, i, j))
+    y.assemble()
+    y.compute()
+    A64 = W.Csr(3000, 2500, Ah.pos, Ah.crd, Ah.vals.astype(np.float32).astype(np.float64))
+    ye, cond = oracle.exact_spmv(A64, xh.astype(np.float32).astype(np.float64))
+    assert oracle.error_report(y.to_numpy(), ye, cond, 1e-5)["ok"]
