@@ -79,7 +79,9 @@ __global__ __launch_bounds__(256) void k_slab_partition(
   const int64_t total = w * (int64_t)m + nnz;
   int64_t d = p * per_part;
   if (d > total || p == parts) d = total;
-  int64_t r = weighted_row(pos, m, w, d);
+  // Part 0 starts at row 0 even when leading rows are empty (with w = 0 the
+  // search would otherwise skip them).
+  int64_t r = p == 0 ? 0 : weighted_row(pos, m, w, d);
   int64_t z = d - w * r;
   if (r < m) {
     if (z > pos[r + 1]) z = pos[r + 1];
@@ -467,19 +469,21 @@ extern "C" int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
   if (parts <= 0) parts = 1024;
   const int32_t slabs = a.n == 0 ? 1 : (int32_t)(((int64_t)a.n + slab_width - 1) / slab_width);
   TH_REQUIRE(slabs <= 1024, "spmv plan: more than 1024 slabs");
-  // Row weight: parts * rows_per_part must cover the rows with a margin,
-  // rows_per_part = (w*m + nnz) / (parts * w) <= kMaxLocalRows - 2.
+  // Row weight w: first w = 0, parts of equal entry counts (so every
+  // workgroup sweeps the slabs at the same pace), accepted when no part
+  // exceeds the LDS row capacity; otherwise the smallest power of two for
+  // which (w*m + nnz) / parts bounds the rows of every part.
   const int64_t cap = kMaxLocalRows - 2;
-  int64_t w = 1;
-  while (w < (1 << 20)) {
-    const int64_t per = (w * (int64_t)a.m + a.nnz + parts - 1) / parts;
-    if (per / w + 1 <= cap) break;
-    w *= 2;
+  int64_t w_bound = 1;
+  while (w_bound < (1 << 20)) {
+    const int64_t per = (w_bound * (int64_t)a.m + a.nnz + parts - 1) / parts;
+    if (per / w_bound + 1 <= cap) break;
+    w_bound *= 2;
   }
-  const int64_t per_part = (w * (int64_t)a.m + a.nnz + parts - 1) / parts;
-  TH_REQUIRE(per_part / w + 1 <= cap,
+  TH_REQUIRE((w_bound * (int64_t)a.m + a.nnz + parts - 1) / parts / w_bound + 1 <= cap,
              "spmv plan: %d parts cannot hold %d rows; use more parts", parts, a.m);
-  TH_REQUIRE(per_part < INT32_MAX, "spmv plan: problem too large");
+  TH_REQUIRE((w_bound * (int64_t)a.m + a.nnz + parts - 1) / parts < INT32_MAX,
+             "spmv plan: problem too large");
 
   auto* p = new taco_hip_spmv_plan();
   p->m = a.m;
@@ -488,7 +492,6 @@ extern "C" int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
   p->parts = parts;
   p->slabs = slabs;
   p->slab_width = slab_width;
-  p->row_weight = w;
   hipStream_t st = current_stream();
   auto alloc = [&](void** q, size_t bytes) {
     return hipMalloc(q, bytes < 16 ? 16 : bytes);
@@ -502,25 +505,32 @@ extern "C" int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
     plan_free(p);
     return fail("spmv plan: device allocation failed");
   }
-  k_slab_partition<<<(parts + 1 + 255) / 256, 256, 0, st>>>(
-      a.m, a.nnz, w, per_part, parts, a.pos, p->part_row, p->part_entry);
   int32_t* maxrows = (int32_t*)workspace(sizeof(int32_t), 3);
   if (!maxrows) {
     plan_free(p);
     return 1;
   }
-  (void)hipMemsetAsync(maxrows, 0, sizeof(int32_t), st);
-  k_max_rows<<<(parts + 255) / 256, 256, 0, st>>>(parts, p->part_row, maxrows);
   int32_t host_max = 0;
-  if (hipMemcpyAsync(&host_max, maxrows, sizeof(int32_t), hipMemcpyDeviceToHost, st) ||
-      hipStreamSynchronize(st)) {
-    plan_free(p);
-    return fail("spmv plan: partition failed");
+  int64_t w = a.nnz > 0 ? 0 : w_bound;
+  for (;;) {
+    const int64_t per_part = (w * (int64_t)a.m + a.nnz + parts - 1) / parts;
+    k_slab_partition<<<(parts + 1 + 255) / 256, 256, 0, st>>>(
+        a.m, a.nnz, w, per_part, parts, a.pos, p->part_row, p->part_entry);
+    (void)hipMemsetAsync(maxrows, 0, sizeof(int32_t), st);
+    k_max_rows<<<(parts + 255) / 256, 256, 0, st>>>(parts, p->part_row, maxrows);
+    if (hipMemcpyAsync(&host_max, maxrows, sizeof(int32_t), hipMemcpyDeviceToHost, st) ||
+        hipStreamSynchronize(st)) {
+      plan_free(p);
+      return fail("spmv plan: partition failed");
+    }
+    if (host_max <= kMaxLocalRows || w >= w_bound) break;
+    w = w == 0 ? 1 : 2 * w;
   }
   if (host_max > kMaxLocalRows) {
     plan_free(p);
     return fail("spmv plan: a part holds %d rows (max %d)", host_max, kMaxLocalRows);
   }
+  p->row_weight = w;
   p->max_local_rows = host_max;
   // Layout: per-slab counts, padded part sizes, their scan, aligned starts.
   int32_t* cnt = nullptr;

@@ -29,6 +29,9 @@ CASES = [
     ("long_rows", lambda: W.csr_from_counts(10, 300000, np.array([0, 250000, 3, 0, 0, 40000, 1, 0, 0, 7], np.int32), 44), 16, 50000),
     ("empty_rows", lambda: W.csr_from_counts(50000, 9000, W.uniform_counts(50000, 0, 2, 45), 45), 40, 2048),
     ("no_entries", lambda: W.Csr(9, 4, np.zeros(10, np.int32), np.zeros(0, np.int32), np.zeros(0)), 4, 0),
+    ("leading_trailing_empty", lambda: W.csr_from_counts(
+        12000, 7000, np.concatenate([np.zeros(3000, np.int32), W.uniform_counts(6000, 0, 40, 46),
+                                     np.zeros(3000, np.int32)]), 46), 16, 1024),
 ]
 
 
