@@ -196,6 +196,16 @@ __global__ __launch_bounds__(256) void k_slab_scatter(
   }
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup p runs on
+// XCD p % 8).  Every workgroup of one XCD starts its sweep at the same slab
+// — so its L2 serves one slab at a time — and different XCDs start at
+// different slabs, which spreads their concurrent misses over the memory
+// side instead of all XCDs fetching the same slab at once.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int32_t first_slab(int32_t part, int32_t slabs) {
+  return (int32_t)((int64_t)(part % kXcds) * slabs / kXcds);
+}
+
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
@@ -253,7 +263,9 @@ __global__ __launch_bounds__(T) void k_slab_spmv(
   __syncthreads();
   const int32_t* o = off + (int64_t)p * (slabs + 1);
   const int32_t* se = seg_end + (int64_t)p * slabs;
-  for (int32_t s = 0; s < slabs; ++s) {
+  const int32_t s_first = first_slab(p, slabs);
+  for (int32_t s0 = 0; s0 < slabs; ++s0) {
+    const int32_t s = s0 + s_first < slabs ? s0 + s_first : s0 + s_first - slabs;
     const int32_t e0 = o[s], e1 = se[s];
     const int32_t nchunks = (e1 - e0 + U - 1) / U;
     const double* xs = x + (int64_t)s * slab_width;
@@ -348,7 +360,9 @@ __global__ __launch_bounds__(T) void k_slab_spmv_wave(
   const int32_t* o = off + (int64_t)p * (slabs + 1);
   const int32_t* se = seg_end + (int64_t)p * slabs;
   const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-  for (int32_t s = 0; s < slabs; ++s) {
+  const int32_t s_first = first_slab(p, slabs);
+  for (int32_t s0 = 0; s0 < slabs; ++s0) {
+    const int32_t s = s0 + s_first < slabs ? s0 + s_first : s0 + s_first - slabs;
     const int32_t e0 = o[s], e1 = se[s];
     const double* xs = x + (int64_t)s * slab_width;
     for (int32_t base = e0; base < e1; base += T * U) {

@@ -45,6 +45,12 @@ DEFAULT_CLASS = {0: SPMV_NAMES.index("thread_per_row"),
                  1: N_SPMV + SPMM_NAMES.index("thread_per_row")}
 
 
+# Classes whose kernels accumulate with atomics (the nnz-split schedules):
+# their results can differ in the last bits from run to run, always within
+# the value tolerance.  Every other class sums in a fixed order.
+ATOMIC_CLASSES = {c for c, name in enumerate(CLASS_NAMES) if "nnz_split" in name}
+
+
 def class_range(op):
     return (0, N_SPMV) if op == 0 else (N_SPMV, N_CLASSES)
 

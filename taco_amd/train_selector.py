@@ -1,8 +1,9 @@
 """Builds the selector's timing database and trains it on the GPU.
 
     python -m taco_amd.train_selector timings --seed 7 --start 0 --count 2000 --out DIR
-    python -m taco_amd.train_selector train --db DIR/timings_*.npz --out taco_amd/data/selector.npz
-    python -m taco_amd.train_selector evaluate --db DIR/heldout_*.npz --weights ...
+    python -m taco_amd.train_selector merge --db 'DIR/timings_s7_*.npz' --out taco_amd/data/selector_train.npz
+    python -m taco_amd.train_selector train --db taco_amd/data/selector_train.npz --out taco_amd/data/selector.npz
+    python -m taco_amd.train_selector evaluate --db taco_amd/data/selector_heldout.npz --weights taco_amd/data/selector.npz
 
 `timings` generates problems (taco_amd.selector.make_problem(seed, i) for i
 in [start, start+count)), computes their features on the GPU and times every
@@ -61,8 +62,19 @@ def load_db(patterns):
         raise SystemExit(f"no timing files match {patterns}")
     cat = {k: np.concatenate([p[k] for p in parts]) for k in ("F", "T", "op", "K", "m", "nnz",
                                                                "structure")}
+    cat["T"] = cat["T"].astype(np.float64)
     ok = np.isfinite(cat["T"].min(axis=1))
     return {k: v[ok] for k, v in cat.items()}
+
+
+def cmd_merge(args):
+    """Concatenates timing files into one compact database (float32)."""
+    db = load_db(args.db)
+    np.savez_compressed(args.out, F=db["F"].astype(np.float32), T=db["T"].astype(np.float32),
+                        op=db["op"].astype(np.int8), K=db["K"].astype(np.int16),
+                        m=db["m"].astype(np.int32), nnz=db["nnz"].astype(np.int64),
+                        structure=db["structure"].astype(np.int8))
+    log(f"wrote {args.out}: {len(db['F'])} problems")
 
 
 def cmd_train(args):
@@ -124,11 +136,15 @@ def main():
     r.add_argument("--lr", type=float, default=2e-3)
     r.add_argument("--wd", type=float, default=1e-5)
     r.add_argument("--seed", type=int, default=0)
+    g = sub.add_parser("merge")
+    g.add_argument("--db", nargs="+", required=True)
+    g.add_argument("--out", required=True)
     e = sub.add_parser("evaluate")
     e.add_argument("--db", nargs="+", required=True)
     e.add_argument("--weights", required=True)
     args = ap.parse_args()
-    {"timings": cmd_timings, "train": cmd_train, "evaluate": cmd_evaluate}[args.cmd](args)
+    {"timings": cmd_timings, "merge": cmd_merge, "train": cmd_train,
+     "evaluate": cmd_evaluate}[args.cmd](args)
 
 
 if __name__ == "__main__":

@@ -124,17 +124,20 @@ def test_selected_schedule_changes_no_result():
         fn = L.taco_hip_spmv_compute if op == 0 else L.taco_hip_spmm_compute
         _lib.check(fn(prob.cd.ptr, prob.a.ptr, prob.bd.ptr, sched))
         torch.cuda.synchronize()
-        assert np.array_equal(got, prob.out.cpu().numpy().astype(np.float64))
+        again = prob.out.cpu().numpy().astype(np.float64)
+        if c not in S.ATOMIC_CLASSES:  # fixed summation order: bit for bit
+            assert np.array_equal(got, again)
         Ah = W.Csr(A.m, A.n, A.pos, A.crd, A.vals if op == 0 else
                    A.vals.astype(np.float32).astype(np.float64))
         inp = prob.inp.cpu().numpy().astype(np.float64)
-        if op == 0:
-            ye, cond = oracle.exact_spmv(Ah, inp)
-            rep = oracle.error_report(got, ye, cond, 1e-12)
-        else:
-            ye, cond = oracle.exact_spmm(Ah, inp)
-            rep = oracle.error_report(got.reshape(A.m, K), ye, cond, 1e-5)
-        assert rep["ok"], (d, rep)
+        for out in (got, again):
+            if op == 0:
+                ye, cond = oracle.exact_spmv(Ah, inp)
+                rep = oracle.error_report(out, ye, cond, 1e-12)
+            else:
+                ye, cond = oracle.exact_spmm(Ah, inp)
+                rep = oracle.error_report(out.reshape(A.m, K), ye, cond, 1e-5)
+            assert rep["ok"], (d, S.CLASS_NAMES[c], rep)
 
 
 @needs_weights
