@@ -240,7 +240,7 @@ def bench_spmv(args, A, x, name, workload_desc):
                              "ms": ms_default, "speedup_vs_default": ms_default / ms},
         "roofline": {"bound": "hbm", "achieved": kernel_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kernel_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_slab_spmv", "kernel_us": kernel_ms * 1e3,
+                     "kernel": "k_slab_spmv_wave", "kernel_us": kernel_ms * 1e3,
                      "kernel_launches_timed": kernel_launches,
                      "algorithmic_bytes_per_launch": nbytes},
         "check_vs_cpu_reference": check,
