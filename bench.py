@@ -533,7 +533,7 @@ def main():
     ap.add_argument("--items", type=int, default=8, help="SpMV merge items per thread")
     ap.add_argument("--parts", type=int, default=1024, help="slab plan parts")
     ap.add_argument("--unroll", type=int, default=0, help="slab plan entries per thread per step (0 = library default)")
-    ap.add_argument("--spmm-items", type=int, default=32)
+    ap.add_argument("--spmm-items", type=int, default=256)
     ap.add_argument("--mfma", type=int, default=1)
     ap.add_argument("--cpu-reps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")

@@ -54,7 +54,7 @@ def spmm_candidates(K):
         out.append((f"rows{rows}", _sched(_lib.GROUP_PER_ROW, 256, group=g, rows=rows)))
     for items in (4, 8, 16):
         out.append((f"nnz_split{items}", _sched(_lib.NNZ_SPLIT, 256, items=items, group=g)))
-    for items in (8, 16, 32, 64):
+    for items in (64, 128, 256, 512):
         out.append((f"merge{items}", _sched(_lib.MERGE_PATH, 256, items=items, group=g)))
     return out
 

@@ -14,10 +14,11 @@ extern "C" void taco_hip_default_schedule(int32_t op, int32_t m, int64_t nnz,
     s.block_size = 256;
     s.items_per_thread = 8;
   } else {
-    // SpMM: merge path per lane team, lanes along k.
+    // SpMM: merge path per lane team, lanes along k, 256 merge items per
+    // team (long teams keep the carries of long rows few).
     s.algo = TACO_HIP_MERGE_PATH;
     s.block_size = 256;
-    s.items_per_thread = 32;
+    s.items_per_thread = 256;
     int g = 8;
     while (g < k && g < 64) g <<= 1;
     s.group_size = g;

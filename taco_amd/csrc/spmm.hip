@@ -184,6 +184,24 @@ __device__ __forceinline__ int64_t group_merge_search(const int32_t* pos,
   return lo;
 }
 
+// Value of lane `idx` of this lane's team of G lanes (idx uniform across
+// the team).  A whole-wave team reads it with v_readlane into a scalar.
+template <int G, typename T>
+__device__ __forceinline__ T team_bcast(T v, int idx) {
+  if constexpr (G == 64) {
+    if constexpr (sizeof(T) == 8) {
+      const uint64_t b = __builtin_bit_cast(uint64_t, v);
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, idx);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), idx);
+      return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    } else {
+      return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), idx));
+    }
+  } else {
+    return __shfl(v, idx, G);
+  }
+}
+
 template <typename TA, typename TC, int G, int CPL>
 __global__ __launch_bounds__(256) void k_spmm_merge(
     int32_t m, int64_t nnz, int32_t K, int32_t items_per_team,
@@ -211,35 +229,42 @@ __global__ __launch_bounds__(256) void k_spmm_merge(
     if (zstop > rend) zstop = rend;
     const int64_t take = zstop - z;
     if (take > 0) partial = true;
-    int64_t zz = z;
-    for (; zz + 4 <= zstop; zz += 4) {
-      int32_t j[4];
-      TC a[4];
+    // The team loads up to G consecutive entries at once (one per lane,
+    // coalesced) and broadcasts them one by one; 8 B-row gathers in flight.
+    for (int64_t zc = z; zc < zstop; zc += G) {
+      const int cnt = (int)(zstop - zc < G ? zstop - zc : G);
+      const int32_t jl = lane < cnt ? stream_load(crd + zc + lane) : 0;
+      const TC al = lane < cnt ? widen(stream_load(vals + zc + lane)) : TC(0);
+      int u = 0;
+      for (; u + 8 <= cnt; u += 8) {
+        int32_t j[8];
+        TC a[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        j[u] = stream_load(crd + zz + u);
-        a[u] = widen(stream_load(vals + zz + u));
+        for (int q = 0; q < 8; ++q) {
+          j[q] = team_bcast<G>(jl, u + q);
+          a[q] = team_bcast<G>(al, u + q);
+        }
+        TC b[8][CPL];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            const int k = lane + c * G;
+            b[q][c] = k < K ? widen(B[(int64_t)j[q] * K + k]) : TC(0);
+          }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) acc[c] += a[q] * b[q][c];
       }
-      TC b[4][CPL];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (; u < cnt; ++u) {
+        const int32_t j = team_bcast<G>(jl, u);
+        const TC a = team_bcast<G>(al, u);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           const int k = lane + c * G;
-          b[u][c] = k < K ? widen(B[(int64_t)j[u] * K + k]) : TC(0);
+          if (k < K) acc[c] += a * widen(B[(int64_t)j * K + k]);
         }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) acc[c] += a[u] * b[u][c];
-    }
-    for (; zz < zstop; ++zz) {
-      const int32_t j = stream_load(crd + zz);
-      const TC a = widen(stream_load(vals + zz));
-#pragma unroll
-      for (int c = 0; c < CPL; ++c) {
-        const int k = lane + c * G;
-        if (k < K) acc[c] += a * widen(B[(int64_t)j * K + k]);
       }
     }
     z = zstop;
@@ -352,7 +377,7 @@ int run_spmm(const CsrView& a, int32_t K, const TA* B, TC* C,
     return fail("spmm: no kernel for G=%d CPL=%d", G, CPL);
   }
   if (algo == TACO_HIP_MERGE_PATH) {
-    int ipt = s.items_per_thread > 0 ? s.items_per_thread : 32;
+    int ipt = s.items_per_thread > 0 ? s.items_per_thread : 256;
     const int64_t total = (int64_t)a.m + a.nnz;
     const int64_t nteams = (total + ipt - 1) / ipt;
     const size_t row_bytes = ((size_t)nteams * sizeof(int32_t) + 255) / 256 * 256;

@@ -154,10 +154,16 @@ def init_params(seed):
 STRUCTURES = ("uniform", "zipf", "banded", "block")
 
 
+def problem_op(seed, index):
+    """The operation (0 SpMV, 1 SpMM) make_problem(seed, index) draws."""
+    return int(np.random.default_rng([seed, index]).integers(0, 2))
+
+
 def make_problem(seed, index, rmin=10_000, rmax=2_000_000, op=None):
     """A seeded random problem: (host Csr, op, K, description dict)."""
     rng = np.random.default_rng([seed, index])
-    op = int(rng.integers(0, 2)) if op is None else op
+    drawn = int(rng.integers(0, 2))
+    op = drawn if op is None else op
     K = int(rng.choice(SPMM_WIDTHS)) if op == 1 else 0
     hi = rmax if op == 0 else max(rmin, min(rmax, (1 << 25) // K))
     m = int(np.exp(rng.uniform(np.log(rmin), np.log(hi))))

@@ -30,24 +30,27 @@ def cmd_timings(args):
     from . import _lib, selector as S
     torch.cuda.set_device(0)
     _lib.check(_lib.lib().taco_hip_set_stream(torch.cuda.current_stream().cuda_stream))
-    n = args.count
+    idx = [args.start + j for j in range(args.count)
+           if args.op < 0 or S.problem_op(args.seed, args.start + j) == args.op]
+    n = len(idx)
     F = np.zeros((n, S.N_FEATURES), np.float32)
     T = np.full((n, S.N_CLASSES), np.inf)
-    meta = {k: np.zeros(n, np.int64) for k in ("op", "K", "m", "nnz", "structure")}
+    meta = {k: np.zeros(n, np.int64) for k in ("op", "K", "m", "nnz", "structure", "index")}
     t0 = time.time()
-    for j in range(n):
-        i = args.start + j
+    for j, i in enumerate(idx):
         A, op, K, d = S.make_problem(args.seed, i, rmin=args.rmin, rmax=args.rmax)
         prob = S.DeviceProblem(A, op, K, seed=i)
         F[j] = S.features(prob.a, op, K)
         T[j] = S.time_classes(prob, reps=args.reps)
         meta["op"][j], meta["K"][j], meta["m"][j], meta["nnz"][j] = op, K, A.m, A.nnz
         meta["structure"][j] = S.STRUCTURES.index(d["structure"])
+        meta["index"][j] = i
         del prob, A
         if (j + 1) % args.progress == 0 or j + 1 == n:
             log(f"[{j + 1}/{n}] {time.time() - t0:.0f}s; last: {d} best "
                 f"{S.CLASS_NAMES[int(np.argmin(T[j]))]} {np.min(T[j]):.3f} ms")
-    out = f"{args.out}/timings_s{args.seed}_{args.start}_{args.start + n}.npz"
+    tag = "" if args.op < 0 else f"_op{args.op}"
+    out = f"{args.out}/timings_s{args.seed}_{args.start}_{args.start + args.count}{tag}.npz"
     np.savez(out, F=F, T=T, seed=args.seed, start=args.start, rmin=args.rmin, rmax=args.rmax,
              **meta)
     log(f"wrote {out} in {time.time() - t0:.0f}s")
@@ -70,6 +73,9 @@ def load_db(patterns):
 def cmd_merge(args):
     """Concatenates timing files into one compact database (float32)."""
     db = load_db(args.db)
+    if args.op >= 0:
+        keep = db["op"] == args.op
+        db = {k: v[keep] for k, v in db.items()}
     np.savez_compressed(args.out, F=db["F"].astype(np.float32), T=db["T"].astype(np.float32),
                         op=db["op"].astype(np.int8), K=db["K"].astype(np.int16),
                         m=db["m"].astype(np.int32), nnz=db["nnz"].astype(np.int64),
@@ -127,6 +133,7 @@ def main():
     t.add_argument("--rmax", type=int, default=2_000_000)
     t.add_argument("--reps", type=int, default=3)
     t.add_argument("--progress", type=int, default=50)
+    t.add_argument("--op", type=int, default=-1, help="only problems of this op (0/1)")
     t.add_argument("--out", default="gpurun_out")
     r = sub.add_parser("train")
     r.add_argument("--db", nargs="+", required=True)
@@ -139,6 +146,7 @@ def main():
     g = sub.add_parser("merge")
     g.add_argument("--db", nargs="+", required=True)
     g.add_argument("--out", required=True)
+    g.add_argument("--op", type=int, default=-1, help="keep only problems of this op")
     e = sub.add_parser("evaluate")
     e.add_argument("--db", nargs="+", required=True)
     e.add_argument("--weights", required=True)
