@@ -18,7 +18,8 @@
 //         stream schedule: the workgroup's products go through LDS in
 //         coalesced chunks and each thread sums one row from LDS (balanced
 //         by entries instead of rows; suits short rows)
-//   SpMM: L lanes per row along k (K / L columns per lane), L in {8,16,32,64}
+//   SpMM: L lanes per row along k (K / L columns per lane), L in {8,16,32,64};
+//         when K = 4L each lane owns 4 adjacent columns (16-byte accesses)
 // A launch may cover a contiguous range of instances (their tasks are
 // contiguous), which the autotuner times one instance at a time with HIP
 // events.  The queue counter is reset by a one-thread kernel, so the pair
@@ -130,6 +131,36 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmm(
     const float* iv = vals + in.ent_off;
     const float* iB = B + in.col_off * (int64_t)K;
     float* iC = C + in.row_off * (int64_t)K;
+    if (K == 4 * L) {
+      // Each lane owns 4 adjacent columns: 16-byte loads of B rows and
+      // stores of C rows, four entries' B rows in flight.
+      typedef float f32x4 __attribute__((ext_vector_type(4)));
+      for (int32_t r = tk.r0 + tid / L; r < tk.r1; r += rows_per_pass) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        const int32_t end = ip[r + 1];
+        int32_t q = ip[r];
+        for (; q + 4 <= end; q += 4) {
+          float a[4];
+          int32_t j[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            a[u] = stream_load(iv + q + u);
+            j[u] = stream_load(ic + q + u);
+          }
+          f32x4 b[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) b[u] = *(const f32x4*)(iB + (int64_t)j[u] * K + 4 * lane);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc += a[u] * b[u];
+        }
+        for (; q < end; ++q) {
+          const float a = stream_load(iv + q);
+          acc += a * *(const f32x4*)(iB + (int64_t)stream_load(ic + q) * K + 4 * lane);
+        }
+        *(f32x4*)(iC + (int64_t)r * K + 4 * lane) = acc;
+      }
+      continue;
+    }
     for (int32_t r = tk.r0 + tid / L; r < tk.r1; r += rows_per_pass) {
       float acc[4] = {0.f, 0.f, 0.f, 0.f};  // K <= 4 * L
       const int32_t end = ip[r + 1];

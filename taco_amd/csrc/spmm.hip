@@ -310,7 +310,17 @@ __global__ __launch_bounds__(256) void k_spmm_merge_fixup(
   while (e < nteams && carry_row[e] == r) ++e;
   for (int32_t k = lane; k < K; k += kWave) {
     TC s = carry_val[t * K + k];
-    for (int64_t q = t + 1; q < e; ++q) s += carry_val[q * K + k];
+    int64_t q = t + 1;
+    // Loads 8 carries ahead, adds them in team order (long rows have
+    // hundreds of carries; the order of the sum stays fixed).
+    for (; q + 8 <= e; q += 8) {
+      TC v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = carry_val[(q + u) * K + k];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; q < e; ++q) s += carry_val[q * K + k];
     C[(int64_t)r * K + k] += s;
   }
 }
