@@ -203,6 +203,32 @@ def bench_spmv(args, A, x, name, workload_desc):
     ms_merge, _ = timer.run(step(s_merge), max(10, args.steps // 2), 3)
     y_merge = y.cpu().numpy()
     ms_default, _ = timer.run(step(s_default), max(5, args.steps // 10), 2)
+    learned = None
+    try:
+        from taco_amd import selector as S
+        sel = S.Selector()
+        c, s_learned, pred_ms = sel.choose(a, 0)
+        ms_learned, _ = timer.run(step(s_learned), max(10, args.steps // 2), 3)
+        learned = {"class": S.CLASS_NAMES[c], "ms": ms_learned, "predicted_ms": pred_ms,
+                   "speedup_vs_default": ms_default / ms_learned}
+    except (OSError, ValueError) as e:  # no trained weights in this tree
+        learned = {"error": str(e)}
+    cold = None
+    if A.nnz < 10_000_000:
+        # Cold: a 512 MB write between launches evicts A and x from the MALL.
+        flush = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        ts = []
+        for _ in range(max(10, args.steps // 5)):
+            flush.fill_(1.0)
+            e0.record()
+            plan.compute(ydsc, xdsc, args.unroll)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        cold = {"ms": float(np.median(ts)), "flush_bytes": flush.numel() * 8}
+        del flush
     nbytes = W.algorithmic_bytes_spmv(A.m, A.n, A.nnz)
     gbs = nbytes / (ms * 1e-3) / 1e9
     check = None
@@ -238,6 +264,8 @@ def bench_spmv(args, A, x, name, workload_desc):
                        "note": "deterministic CSR merge-path kernel, no plan"},
         "default_schedule": {"schedule": "thread_per_row (taco split(i,256))",
                              "ms": ms_default, "speedup_vs_default": ms_default / ms},
+        "learned_schedule": learned,
+        "cold": cold,
         "roofline": {"bound": "hbm", "achieved": kernel_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kernel_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_slab_spmv_wave", "kernel_us": kernel_ms * 1e3,

@@ -36,6 +36,8 @@ def cmd_timings(args):
     F = np.zeros((n, S.N_FEATURES), np.float32)
     T = np.full((n, S.N_CLASSES), np.inf)
     meta = {k: np.zeros(n, np.int64) for k in ("op", "K", "m", "nnz", "structure", "index")}
+    tag = "" if args.op < 0 else f"_op{args.op}"
+    out = f"{args.out}/timings_s{args.seed}_{args.start}_{args.start + args.count}{tag}.npz"
     t0 = time.time()
     for j, i in enumerate(idx):
         A, op, K, d = S.make_problem(args.seed, i, rmin=args.rmin, rmax=args.rmax)

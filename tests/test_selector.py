@@ -112,6 +112,11 @@ def test_problems_are_seeded_and_cover_structures():
     assert seen == set(S.STRUCTURES)
 
 
+def test_problem_op_matches_generated_problem():
+    for i in range(40):
+        assert S.problem_op(7, i) == S.make_problem(7, i, rmin=1000, rmax=3000)[1]
+
+
 def test_class_space():
     assert len(S.CLASS_NAMES) == 24
     assert S.DEFAULT_CLASS[0] == 0 and S.DEFAULT_CLASS[1] == 13
