@@ -51,10 +51,9 @@ def cmd_timings(args):
         if (j + 1) % args.progress == 0 or j + 1 == n:
             log(f"[{j + 1}/{n}] {time.time() - t0:.0f}s; last: {d} best "
                 f"{S.CLASS_NAMES[int(np.argmin(T[j]))]} {np.min(T[j]):.3f} ms")
-    tag = "" if args.op < 0 else f"_op{args.op}"
-    out = f"{args.out}/timings_s{args.seed}_{args.start}_{args.start + args.count}{tag}.npz"
-    np.savez(out, F=F, T=T, seed=args.seed, start=args.start, rmin=args.rmin, rmax=args.rmax,
-             **meta)
+            # the problems timed so far, rewritten at every progress line
+            np.savez(out, F=F[:j + 1], T=T[:j + 1], seed=args.seed, start=args.start,
+                     rmin=args.rmin, rmax=args.rmax, **{k: v[:j + 1] for k, v in meta.items()})
     log(f"wrote {out} in {time.time() - t0:.0f}s")
 
 
