@@ -145,16 +145,18 @@ def gpu_setup():
     return torch, L, _lib
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration
-    recorded with it), or None when no summary covers the kernel."""
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration recorded with it), or None when no summary
+    covers that kernel on that workload."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         import json as _json
         with open(path) as f:
             data = _json.load(f)
-        return data["kernels"][kernel]["hbm_bytes_per_launch"]
+        entry = data["kernels"][kernel]
+        return entry["hbm_bytes_per_launch"] if entry["workload"] == workload else None
     except (OSError, KeyError, ValueError):
         return None
 
@@ -243,7 +245,7 @@ def bench_spmv(args, A, x, name, workload_desc):
                "kind": kind, "ms": cms, "median_ms": float(np.median(cpu_ts)),
                "sample": f"whole workload, {len(cpu_ts)} timed compute() calls "
                          f"(10% trimmed mean), JIT excluded, inputs in host DRAM"}
-    traffic = profiled_traffic("k_slab_spmv_wave")
+    traffic = profiled_traffic("k_slab_spmv_wave", name)
     kernel_gbs = nbytes / (kernel_ms * 1e-3) / 1e9
     return {
         "metric": "SpMV HBM GB/s (% of 8 TB/s), SpMM GFLOP/s: learned vs default schedule at tol",
