@@ -158,7 +158,7 @@ int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,
 
 /* Slab-tiled SpMV plan (f64).  A storage conversion of a CSR matrix for
  * SpMV whose x exceeds one XCD's L2: columns split into slabs of slab_width
- * (<= 2^18; 0 = 2^18, 2 MB of x), rows into `parts` workgroup parts (0 = 1024)
+ * (<= 2^18; 0 = 2^17, 1 MB of x), rows into `parts` workgroup parts (0 = 1024)
  * whose entries are reordered slab by slab, so that resident workgroups sweep
  * the slabs in step and gather x from L2.  No reference counterpart: the
  * reference's nearest analogue is packing a tensor into another format

@@ -4,7 +4,7 @@
 // Random gathers x[crd] over a vector larger than the 4 MB L2 of an XCD are
 // served from the memory-side MALL at a fraction of the L2 hit rate, and they
 // dominate SpMV on matrices with scattered columns.  The plan splits the
-// columns into slabs of `slab_width` (2^18 columns = 2 MB of f64 by default)
+// columns into slabs of `slab_width` (2^17 columns = 1 MB of f64 by default)
 // and the rows into P parts sized so that one part's row partial sums fit in
 // LDS and all parts are resident at once.  Inside each part the entries are
 // reordered by slab (stable, so rows stay ordered within a slab).  Every
@@ -53,6 +53,8 @@ constexpr int kColBits = 18;
 constexpr uint32_t kColMask = (1u << kColBits) - 1;
 constexpr int kPlanThreads = 256;
 constexpr int kDefaultUnroll = 1;
+// 2^17 columns = 1 MB of f64 x per slab (a quarter of an XCD's L2).
+constexpr int kDefaultSlabWidth = 1 << 17;
 // Segment start alignment (entries); the largest per-thread chunk.
 constexpr int kAlign = 8;
 // Row partials live in dynamic LDS sized by the plan's largest part; this
@@ -478,7 +480,7 @@ extern "C" int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
   CsrView a;
   if (int rc = view_csr(A, "A", &a)) return rc;
   TH_REQUIRE(a.vt == VType::F64, "spmv plan: f64 values only");
-  if (slab_width <= 0) slab_width = 1 << kColBits;
+  if (slab_width <= 0) slab_width = kDefaultSlabWidth;
   TH_REQUIRE(slab_width <= (1 << kColBits), "spmv plan: slab width above 2^18");
   if (parts <= 0) parts = 1024;
   const int32_t slabs = a.n == 0 ? 1 : (int32_t)(((int64_t)a.n + slab_width - 1) / slab_width);
