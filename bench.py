@@ -536,6 +536,62 @@ def bench_selector(args):
     }
 
 
+def bench_codegen(args):
+    """Modules emitted by CodeGen_HIP (the reference's GPU schedules) beside
+    the hand-written kernels of the same schedule family, on the 2M-row
+    power-law matrix: SpMV f64 and SpMM f64 with K = 64."""
+    torch, L, _lib = gpu_setup()
+    from taco_amd import codegen, descriptors as D, workloads as W
+    A = W.powerlaw_2m()
+    x = W.uniform(A.n, 2, W.STREAM_X)
+    pos = torch.from_numpy(A.pos).cuda()
+    crd = torch.from_numpy(A.crd).cuda()
+    vals = torch.from_numpy(A.vals).cuda()
+    a = D.csr(pos, crd, vals, A.shape)
+    xd = D.dense(torch.from_numpy(x).cuda())
+    y = D.dense(torch.empty(A.m, dtype=torch.float64, device="cuda"))
+    K = 64
+    Bt = torch.rand((A.n, K), dtype=torch.float64, device="cuda")
+    Ct = torch.empty((A.m, K), dtype=torch.float64, device="cuda")
+    b, c = D.dense(Bt), D.dense(Ct)
+    timer = Timer(torch)
+    rows = {}
+    for name in ("spmv_default_f64", "spmv_rows_f64", "spmv_rows64_f64", "spmv_nnz_f64",
+                 "spmv_merge_f64", "spmm_default_f64", "spmm_nnz_f64"):
+        k = codegen.Kernel(codegen.standard(name))
+        descs = (y, a, xd) if name.startswith("spmv") else (c, a, b)
+        reps = 3 if "default" in name else args.steps
+        ms, _ = timer.run(lambda: k.compute(*descs), reps, 1)
+        rows[name] = {"ms": ms}
+    hand = {"spmv_thread_per_row": sched(_lib, _lib.THREAD_PER_ROW, 256),
+            "spmv_group32": sched(_lib, _lib.GROUP_PER_ROW, 256, group=32),
+            "spmv_nnz_split8": sched(_lib, _lib.NNZ_SPLIT, 256, 8),
+            "spmv_merge8": sched(_lib, _lib.MERGE_PATH, 256, 8)}
+    for name, s_ in hand.items():
+        ms, _ = timer.run(lambda: _lib.check(L.taco_hip_spmv_compute(y.ptr, a.ptr, xd.ptr, s_)),
+                          3 if "thread" in name else args.steps, 1)
+        rows["hand_" + name] = {"ms": ms}
+    s_mm = sched(_lib, _lib.MERGE_PATH, 256, 256, 64)
+    ms, _ = timer.run(lambda: _lib.check(L.taco_hip_spmm_compute(c.ptr, a.ptr, b.ptr, s_mm)),
+                      args.steps, 1)
+    rows["hand_spmm_merge256"] = {"ms": ms}
+    nb = W.algorithmic_bytes_spmv(A.m, A.n, A.nnz)
+    for k_, v in rows.items():
+        if "spmv" in k_:
+            v["gbs"] = nb / (v["ms"] * 1e-3) / 1e9
+        else:
+            v["gflops"] = 2.0 * A.nnz * K / (v["ms"] * 1e-3) / 1e9
+    best = rows["spmv_merge_f64"]
+    return {
+        "metric": "CodeGen_HIP modules vs hand-written kernels (power-law 2M)",
+        "value": best["gbs"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": best["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seed-generated)",
+        "config": {"workload": "codegen_powerlaw_2m", "nnz": A.nnz, "k": K},
+        "kernels": rows, "roofline": None, "cpu_baseline": None,
+    }
+
+
 def copy_calibration(args):
     """A plain device copy of known size (2 GiB read + 2 GiB written per
     launch), run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE into
@@ -596,6 +652,8 @@ def main():
         out = copy_calibration(args)
     elif w == "selector":
         out = bench_selector(args)
+    elif w == "codegen":
+        out = bench_codegen(args)
     elif w in ("batched_spmv", "batched_spmm"):
         out = bench_batched(args, 0 if w == "batched_spmv" else 1)
     else:

@@ -307,7 +307,12 @@ typedef enum taco_hip_gen_family {
   TACO_HIP_GEN_DEFAULT = 0,
   TACO_HIP_GEN_WARP_ROWS = 1,
   TACO_HIP_GEN_NNZ_SPLIT = 2,
-  TACO_HIP_GEN_TWO_PHASE = 3
+  TACO_HIP_GEN_TWO_PHASE = 3,
+  /* SpMV only: merge-path load balancing over row ends and entries
+   * (nnz_per_thread merge items per thread, atomics at shared rows).  No
+   * reference schedule; the facade maps fuse(i,j,f) split without pos()
+   * onto it. */
+  TACO_HIP_GEN_MERGE = 4
 } taco_hip_gen_family;
 
 typedef struct taco_hip_gen_spec {

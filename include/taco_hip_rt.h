@@ -84,6 +84,20 @@ __device__ __host__ inline int32_t taco_hip_binarySearchBefore(
   return lo;
 }
 
+/* Merge path over (row ends, entries) of a CSR pos array: the number of
+ * rows r in [0, m) whose end item lies before merge position d, i.e. whose
+ * pos[r + 1] + r + 1 <= d.  Row i = result is the row that position d falls
+ * in; d - i entries precede it. */
+__device__ __host__ inline int32_t taco_hip_mergePathSearch(const int32_t* pos,
+                                                            int32_t m, int64_t d) {
+  int32_t lo = 0, hi = m;
+  while (lo < hi) {
+    const int32_t mid = lo + (hi - lo) / 2;
+    if ((int64_t)pos[mid + 1] + mid + 1 <= d) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
 /* results[b] = binarySearchBefore(array, lo, hi, b * per_block) for b in
  * [0, num_blocks]: the row where each block's entry chunk starts. */
 __global__ __launch_bounds__(256) void taco_hip_binarySearchBeforeBlock(

@@ -151,7 +151,7 @@ class GenSpec(Structure):
 
 
 OP_SPMV, OP_SPMM, OP_SPGEMM = range(3)
-GEN_DEFAULT, GEN_WARP_ROWS, GEN_NNZ_SPLIT, GEN_TWO_PHASE = range(4)
+GEN_DEFAULT, GEN_WARP_ROWS, GEN_NNZ_SPLIT, GEN_TWO_PHASE, GEN_MERGE = range(5)
 
 
 def generate(spec, what=0):

@@ -253,6 +253,18 @@ def _recognise(stmt, kind, a):
             block = s1[4] // per_thread
             warp = s2[4] // per_thread
             return (_lib.GEN_NNZ_SPLIT, block, per_thread, warp, 0, 0)
+    if kind == "spmv" and ops == ["fuse", "split", "split", "parallelize", "parallelize"]:
+        # fuse(i,j,f) split over rows and entries together, without pos():
+        # merge-path load balancing (K merge items per thread).
+        fuse = next(c for c in cmds if c[0] == "fuse")
+        s1, s2 = sp
+        ok = (fuse[1] is i and fuse[2] is j and s1[1] is fuse[3] and s2[1] is s1[3]
+              and units.get(s1[2], (None,))[0] == ParallelUnit.GPUBlock
+              and units.get(s2[2], (None, None)) == (ParallelUnit.GPUThread,
+                                                    OutputRaceStrategy.Atomics))
+        if ok:
+            per_thread = s2[4]
+            return (_lib.GEN_MERGE, s1[4] // per_thread, per_thread, 0, 0, 0)
     if kind == "spmv" and ops == ["split", "split", "pos", "split", "parallelize",
                                   "parallelize", "parallelize"]:
         pos = next(c for c in cmds if c[0] == "pos")

@@ -219,6 +219,52 @@ Stmt spmv_nnz_split(const taco_hip_gen_spec& s, DType vt, const Csr& A,
       LoopKind::Serial, ParallelUnit::GPUBlock, OutputRaceStrategy::IgnoreRaces);
 }
 
+// Merge-path load balancing (no reference schedule; the HIP backend's
+// reading of fuse(i,j,f) split over rows and entries together): the merge
+// items are the m row ends and the nnz entries, K per thread.  A thread
+// finds its start (row i, entry fA) with taco_hip_mergePathSearch, adds
+// entries into t and, at each row end, adds t into y[i] atomically (the
+// first and last rows of a thread may be shared with its neighbours); the
+// thread's last, unfinished row is flushed wave-aggregated.
+Stmt spmv_merge(const taco_hip_gen_spec& s, DType vt, const Csr& A, const Dense& x,
+                const Dense& y, std::vector<Stmt>& pre) {
+  const int B = s.block_size > 0 ? s.block_size : 256;
+  const int K = s.nnz_per_thread > 0 ? s.nnz_per_thread : 8;
+  Expr nnz = var("A_nnz", DType::Int32), total = var("merge_items", DType::Int64);
+  Expr nb = var("num_blocks", DType::Int32);
+  pre.push_back(nnz_decl(A, nnz));
+  pre.push_back(decl(total, add(i64(A.dim1), i64(nnz))));
+  pre.push_back(decl(nb, cast(DType::Int32, div(add(total, lit(B * K - 1, DType::Int64)),
+                                                 lit(B * K, DType::Int64)))));
+  pre.push_back(launch("taco_hip_fill", {y.vals, i64(A.dim1), zero(vt)}));
+  Expr block_v = var("block", DType::Int32), thread = var("thread", DType::Int32);
+  Expr d0 = var("d0", DType::Int64), q = var("q", DType::Int32);
+  Expr i = var("i", DType::Int32), fA = var("fA", DType::Int32), t = var("tval", vt);
+  Stmt step = if_then(
+      lt(add(d0, i64(q)), total),
+      if_then(lt(fA, load(A.pos, add(i, i32(1)))),
+              block({assign(t, add(t, mul(load(A.vals, fA), load(x.vals, load(A.crd, fA))))),
+                     assign(fA, add(fA, i32(1)))}),
+              block({store_add(y.vals, i, t, true, ParallelUnit::GPUThread),
+                     assign(t, zero(vt)), assign(i, add(i, i32(1)))})));
+  Stmt items = for_loop(q, i32(0), i32(K), i32(1), step, LoopKind::Serial,
+                        ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces, K);
+  Stmt thread_body = block({
+      decl(d0, mul(add(mul(i64(block_v), lit(B, DType::Int64)), i64(thread)),
+                   lit(K, DType::Int64))),
+      if_then(lt(d0, total),
+              block({decl(i, call("taco_hip_mergePathSearch", {A.pos, A.dim1, d0},
+                                  DType::Int32)),
+                     decl(fA, cast(DType::Int32, sub(d0, i64(i)))),
+                     decl(t, zero(vt)), items,
+                     if_then(lt(i, A.dim1),
+                             store_add(y.vals, i, t, true, ParallelUnit::GPUWarp))}))});
+  return for_loop(block_v, i32(0), nb, i32(1),
+                  for_loop(thread, i32(0), i32(B), i32(1), thread_body, LoopKind::Serial,
+                           ParallelUnit::GPUThread, OutputRaceStrategy::Atomics),
+                  LoopKind::Serial, ParallelUnit::GPUBlock, OutputRaceStrategy::IgnoreRaces);
+}
+
 std::vector<Stmt> lower_spmv(const taco_hip_gen_spec& s, DType vt) {
   Csr A = unpack_csr("A", vt);
   Dense x = unpack_dense("x", vt, 1), y = unpack_dense("y", vt, 1);
@@ -228,6 +274,7 @@ std::vector<Stmt> lower_spmv(const taco_hip_gen_spec& s, DType vt) {
     case TACO_HIP_GEN_DEFAULT: k = spmv_default(s, vt, A, x, y); break;
     case TACO_HIP_GEN_WARP_ROWS: k = spmv_warp_rows(s, vt, A, x, y); break;
     case TACO_HIP_GEN_NNZ_SPLIT: k = spmv_nnz_split(s, vt, A, x, y, pre); break;
+    case TACO_HIP_GEN_MERGE: k = spmv_merge(s, vt, A, x, y, pre); break;
     default: throw std::runtime_error("SpMV: unsupported schedule family");
   }
   pre.push_back(k);

@@ -49,6 +49,18 @@ def schedule_spmv_thread_per_row_gpu(stmt, A, precomputed, i, j, BLOCK_SIZE=256)
             .parallelize(thread, PU.GPUThread, ORS.NoRaces))
 
 
+def schedule_spmv_merge_gpu(stmt, A, precomputed, i, j, ITEMS_PER_THREAD=8, BLOCK_SIZE=256):
+    """The HIP backend's merge-path schedule: the fused (i,j) space split
+    directly (rows and entries together), without pos()."""
+    f, f1, item = IndexVar("f"), IndexVar("f1"), IndexVar("item")
+    block, thread = IndexVar("block"), IndexVar("thread")
+    return (stmt.fuse(i, j, f)
+            .split(f, block, f1, ITEMS_PER_THREAD * BLOCK_SIZE)
+            .split(f1, thread, item, ITEMS_PER_THREAD)
+            .parallelize(block, PU.GPUBlock, ORS.IgnoreRaces)
+            .parallelize(thread, PU.GPUThread, ORS.Atomics))
+
+
 def schedule_spmm_gpu(stmt, A, precomputed, i, j, k, NNZ_PER_WARP=8, BLOCK_SIZE=256):
     NNZ_PER_TB = NNZ_PER_WARP * (BLOCK_SIZE // WARP_SIZE)
     f, fpos, block, fpos1, warp, nnz = (IndexVar("f"), IndexVar("fpos"), IndexVar("block"),
