@@ -235,6 +235,14 @@ Stmt ret(Expr v) {
   return s;
 }
 Stmt brk() { return ms(SK::Break); }
+Stmt allocate_shared(Expr v, Expr n) {
+  auto s = ms(SK::Allocate);
+  s->var = v;
+  s->num_elements = n;
+  s->shared = true;
+  return s;
+}
+Stmt sync_threads() { return ms(SK::SyncThreads); }
 Stmt sort(Expr arr, Expr size) {
   auto s = ms(SK::Sort);
   s->arr = arr;
@@ -344,8 +352,8 @@ void dump(std::ostream& os, const Stmt& s, int ind) {
       break;
     case SK::Allocate:
       os << pad << "allocate " << s->var->name << "[" << to_string(s->num_elements)
-         << "]" << (s->clear ? " clear" : "") << (s->on_device ? " device" : " host")
-         << "\n";
+         << "]" << (s->clear ? " clear" : "")
+         << (s->shared ? " shared" : s->on_device ? " device" : " host") << "\n";
       break;
     case SK::Free:
       os << pad << "free " << s->var->name << "\n";

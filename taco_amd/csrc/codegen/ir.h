@@ -93,6 +93,7 @@ struct StmtNode {
   bool is_realloc = false;
   bool on_device = true;
   int scratch_slot = -1;  // >= 0: library workspace slot, not freed
+  bool shared = false;    // in a kernel: one LDS array per workgroup
   // Function
   std::string name;
   std::vector<Expr> outputs, inputs;
@@ -154,6 +155,11 @@ Stmt while_loop(Expr c, Stmt body);
 Stmt allocate(Expr v, Expr n, bool clear = false, bool on_device = true);
 // Device scratch from the library's reusable workspace (slot 4..7).
 Stmt allocate_scratch(Expr v, Expr n, int slot, bool clear = false);
+// A workgroup-shared (LDS) array of literal size inside a GPUBlock loop
+// (the reference's __shared__ temporaries, codegen_cuda.cpp:1023-1070).
+Stmt allocate_shared(Expr v, Expr n);
+// Workgroup barrier.
+Stmt sync_threads();
 Stmt free_stmt(Expr v);
 Stmt comment(const std::string& text);
 Stmt ret(Expr v);

@@ -221,11 +221,13 @@ Stmt spmv_nnz_split(const taco_hip_gen_spec& s, DType vt, const Csr& A,
 
 // Merge-path load balancing (no reference schedule; the HIP backend's
 // reading of fuse(i,j,f) split over rows and entries together): the merge
-// items are the m row ends and the nnz entries, K per thread.  A thread
-// finds its start (row i, entry fA) with taco_hip_mergePathSearch, adds
-// entries into t and, at each row end, adds t into y[i] atomically (the
-// first and last rows of a thread may be shared with its neighbours); the
-// thread's last, unfinished row is flushed wave-aggregated.
+// items are the m row ends and the nnz entries, K per thread, B*K per
+// workgroup.  The workgroup first computes the products of its entry range
+// into LDS with coalesced loads (one entry per thread per step), then each
+// thread finds its start (row i, entry fA) with taco_hip_mergePathSearch,
+// adds its entries' products from LDS and, at each row end, adds the sum
+// into y[i] atomically (a thread's first and last rows may be shared with
+// its neighbours); its last, unfinished row is flushed wave-aggregated.
 Stmt spmv_merge(const taco_hip_gen_spec& s, DType vt, const Csr& A, const Dense& x,
                 const Dense& y, std::vector<Stmt>& pre) {
   const int B = s.block_size > 0 ? s.block_size : 256;
@@ -238,31 +240,45 @@ Stmt spmv_merge(const taco_hip_gen_spec& s, DType vt, const Csr& A, const Dense&
                                                  lit(B * K, DType::Int64)))));
   pre.push_back(launch("taco_hip_fill", {y.vals, i64(A.dim1), zero(vt)}));
   Expr block_v = var("block", DType::Int32), thread = var("thread", DType::Int32);
+  Expr D0 = var("block_d0", DType::Int64), D1 = var("block_d1", DType::Int64);
+  Expr z0 = var("block_z0", DType::Int32), z1 = var("block_z1", DType::Int32);
+  Expr e = var("e", DType::Int32), prod = var("prod", vt, true);
   Expr d0 = var("d0", DType::Int64), q = var("q", DType::Int32);
   Expr i = var("i", DType::Int32), fA = var("fA", DType::Int32), t = var("tval", vt);
+  auto search = [&](Expr d) {
+    return call("taco_hip_mergePathSearch", {A.pos, A.dim1, d}, DType::Int32);
+  };
+  Stmt stage = for_loop(
+      thread, i32(0), i32(B), i32(1),
+      for_loop(e, add(z0, thread), z1, i32(B),
+               store(prod, sub(e, z0), mul(load(A.vals, e), load(x.vals, load(A.crd, e))))),
+      LoopKind::Serial, ParallelUnit::GPUThread, OutputRaceStrategy::IgnoreRaces);
   Stmt step = if_then(
       lt(add(d0, i64(q)), total),
       if_then(lt(fA, load(A.pos, add(i, i32(1)))),
-              block({assign(t, add(t, mul(load(A.vals, fA), load(x.vals, load(A.crd, fA))))),
-                     assign(fA, add(fA, i32(1)))}),
+              block({assign(t, add(t, load(prod, sub(fA, z0)))), assign(fA, add(fA, i32(1)))}),
               block({store_add(y.vals, i, t, true, ParallelUnit::GPUThread),
                      assign(t, zero(vt)), assign(i, add(i, i32(1)))})));
   Stmt items = for_loop(q, i32(0), i32(K), i32(1), step, LoopKind::Serial,
                         ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces, K);
-  Stmt thread_body = block({
-      decl(d0, mul(add(mul(i64(block_v), lit(B, DType::Int64)), i64(thread)),
-                   lit(K, DType::Int64))),
-      if_then(lt(d0, total),
-              block({decl(i, call("taco_hip_mergePathSearch", {A.pos, A.dim1, d0},
-                                  DType::Int32)),
-                     decl(fA, cast(DType::Int32, sub(d0, i64(i)))),
-                     decl(t, zero(vt)), items,
-                     if_then(lt(i, A.dim1),
-                             store_add(y.vals, i, t, true, ParallelUnit::GPUWarp))}))});
-  return for_loop(block_v, i32(0), nb, i32(1),
-                  for_loop(thread, i32(0), i32(B), i32(1), thread_body, LoopKind::Serial,
-                           ParallelUnit::GPUThread, OutputRaceStrategy::Atomics),
-                  LoopKind::Serial, ParallelUnit::GPUBlock, OutputRaceStrategy::IgnoreRaces);
+  Stmt consume = for_loop(
+      thread, i32(0), i32(B), i32(1),
+      block({decl(d0, add(D0, mul(i64(thread), lit(K, DType::Int64)))),
+             if_then(lt(d0, total),
+                     block({decl(i, search(d0)), decl(fA, cast(DType::Int32, sub(d0, i64(i)))),
+                            decl(t, zero(vt)), items,
+                            if_then(lt(i, A.dim1),
+                                    store_add(y.vals, i, t, true, ParallelUnit::GPUWarp))}))}),
+      LoopKind::Serial, ParallelUnit::GPUThread, OutputRaceStrategy::Atomics);
+  Stmt body = block({
+      allocate_shared(prod, i32(B * K)),
+      decl(D0, mul(i64(block_v), lit(B * K, DType::Int64))),
+      decl(D1, min(add(D0, lit(B * K, DType::Int64)), total)),
+      decl(z0, cast(DType::Int32, sub(D0, i64(search(D0))))),
+      decl(z1, cast(DType::Int32, sub(D1, i64(search(D1))))),
+      stage, sync_threads(), consume});
+  return for_loop(block_v, i32(0), nb, i32(1), body, LoopKind::Serial,
+                  ParallelUnit::GPUBlock, OutputRaceStrategy::IgnoreRaces);
 }
 
 std::vector<Stmt> lower_spmv(const taco_hip_gen_spec& s, DType vt) {
