@@ -308,10 +308,12 @@ typedef enum taco_hip_gen_family {
   TACO_HIP_GEN_WARP_ROWS = 1,
   TACO_HIP_GEN_NNZ_SPLIT = 2,
   TACO_HIP_GEN_TWO_PHASE = 3,
-  /* SpMV only: merge-path load balancing over row ends and entries
-   * (nnz_per_thread merge items per thread, atomics at shared rows).  No
-   * reference schedule; the facade maps fuse(i,j,f) split without pos()
-   * onto it. */
+  /* Merge-path load balancing over row ends and entries; no reference
+   * schedule (the facade maps fuse(i,j,f) split without pos() onto it).
+   * SpMV: nnz_per_thread merge items per thread, products staged in LDS.
+   * SpMM: teams of warp_size lanes along k with `bound` columns per lane,
+   * nnz_per_thread merge items per team, entries loaded by the team and
+   * broadcast.  Rows shared between threads/teams are added atomically. */
   TACO_HIP_GEN_MERGE = 4
 } taco_hip_gen_family;
 

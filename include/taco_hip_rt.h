@@ -98,6 +98,13 @@ __device__ __host__ inline int32_t taco_hip_mergePathSearch(const int32_t* pos,
   return lo;
 }
 
+/* Value of lane `idx` of the caller's team of `width` lanes (idx uniform
+ * across the team). */
+template <typename T>
+__device__ inline T taco_hip_team_bcast(T v, int32_t idx, int32_t width) {
+  return __shfl(v, idx, width);
+}
+
 /* results[b] = binarySearchBefore(array, lo, hi, b * per_block) for b in
  * [0, num_blocks]: the row where each block's entry chunk starts. */
 __global__ __launch_bounds__(256) void taco_hip_binarySearchBeforeBlock(

@@ -296,6 +296,24 @@ def _recognise(stmt, kind, a):
             warp = s3[4]
             block = (s1[4] // per_warp) * warp
             return (_lib.GEN_NNZ_SPLIT, block, per_warp, warp, 0, bnd[3])
+    if kind == "spmm" and ops == ["fuse", "split", "split", "split", "bound",
+                                  "parallelize", "parallelize", "parallelize"]:
+        # scheduleSpMMGPU's form without pos(): the fused (i,j) space split
+        # over rows and entries together, merge-path teams along k.
+        fuse = next(c for c in cmds if c[0] == "fuse")
+        bnd = next(c for c in cmds if c[0] == "bound")
+        s1, s2, s3 = sp
+        ok = (fuse[1] is i and fuse[2] is j and s1[1] is fuse[3] and s2[1] is s1[3]
+              and bnd[1] is s3[2]
+              and units.get(s1[2], (None,))[0] == ParallelUnit.GPUBlock
+              and units.get(s2[2], (None,))[0] == ParallelUnit.GPUWarp
+              and units.get(s3[3], (None, None)) == (ParallelUnit.GPUThread,
+                                                    OutputRaceStrategy.Atomics))
+        if ok:
+            per_team = s2[4]
+            warp = s3[4]
+            block = (s1[4] // per_team) * warp
+            return (_lib.GEN_MERGE, block, per_team, warp, 0, bnd[3])
     raise TacoError(f"schedule not supported by the HIP backend: {ops}")
 
 

@@ -9,7 +9,7 @@ from . import _lib
 
 # Named schedules: the reference's GPU schedules at their default parameters
 # (test/tests-scheduling-eval.cpp), the f32 forms, and the HIP backend's
-# merge-path SpMV.
+# merge-path SpMV and SpMM.
 STANDARD_SPECS = {
     "spmv_default_f64": (_lib.OP_SPMV, _lib.GEN_DEFAULT, 64, 256, 0, 0, 0, 0),
     "spmv_rows_f64": (_lib.OP_SPMV, _lib.GEN_WARP_ROWS, 64, 256, 0, 32, 1, 0),
@@ -21,6 +21,7 @@ STANDARD_SPECS = {
     "spmm_default_f64": (_lib.OP_SPMM, _lib.GEN_DEFAULT, 64, 256, 0, 0, 0, 0),
     "spmm_nnz_f64": (_lib.OP_SPMM, _lib.GEN_NNZ_SPLIT, 64, 256, 8, 32, 0, 4),
     "spmm_nnz_f32": (_lib.OP_SPMM, _lib.GEN_NNZ_SPLIT, 32, 256, 8, 32, 0, 4),
+    "spmm_merge_f64": (_lib.OP_SPMM, _lib.GEN_MERGE, 64, 256, 256, 64, 0, 2),
     "spgemm_f64": (_lib.OP_SPGEMM, _lib.GEN_TWO_PHASE, 64, 256, 0, 0, 0, 0),
 }
 

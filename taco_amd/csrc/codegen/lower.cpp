@@ -393,6 +393,94 @@ Stmt spmm_nnz_split(const taco_hip_gen_spec& s, DType vt, const Csr& A,
       LoopKind::Serial, ParallelUnit::GPUBlock);
 }
 
+// Merge-path SpMM (no reference schedule): teams of W lanes along k, D
+// columns per lane (k = dv*W + lane, the bound of scheduleSpMMGPU), P merge
+// items (row ends and entries) per team.  Inside a row segment the team
+// loads up to W entries at once, one per lane, and broadcasts them
+// (taco_hip_team_bcast); each lane gathers its columns of the B rows.  The
+// team's first row may have started in the previous team and its last row
+// may continue in the next: those are added with atomics, every row that
+// starts and ends inside the team is stored.
+Stmt spmm_merge(const taco_hip_gen_spec& s, DType vt, const Csr& A, const Dense& Bm,
+                const Dense& C, std::vector<Stmt>& pre) {
+  const int BS = s.block_size > 0 ? s.block_size : 256;
+  const int W = s.warp_size > 0 ? s.warp_size : 64;
+  const int P = s.nnz_per_thread > 0 ? s.nnz_per_thread : 256;
+  const int D = s.bound > 0 ? s.bound : 1;
+  if (BS % W || W > 64 || (W & (W - 1)))
+    throw std::runtime_error("MERGE: lanes per team must be a power of two <= 64 dividing the block");
+  const int teams = BS / W;
+  Expr nnz = var("A_nnz", DType::Int32), total = var("merge_items", DType::Int64);
+  Expr nb = var("num_blocks", DType::Int32);
+  pre.push_back(nnz_decl(A, nnz));
+  pre.push_back(decl(total, add(i64(A.dim1), i64(nnz))));
+  pre.push_back(decl(nb, cast(DType::Int32, div(add(total, lit((int64_t)teams * P - 1, DType::Int64)),
+                                                 lit((int64_t)teams * P, DType::Int64)))));
+  pre.push_back(launch("taco_hip_fill", {C.vals, mul(i64(C.dim1), i64(C.dim2)), zero(vt)}));
+  Expr block_v = var("block", DType::Int32), warp = var("warp", DType::Int32);
+  Expr lane = var("lane", DType::Int32), dv = var("dense_val", DType::Int32);
+  Expr d = var("d", DType::Int64), dend = var("dend", DType::Int64);
+  Expr i = var("i", DType::Int32), i_first = var("i_first", DType::Int32);
+  Expr fA = var("fA", DType::Int32), rend = var("rend", DType::Int32);
+  Expr zstop = var("zstop", DType::Int32), zc = var("zc", DType::Int32);
+  Expr cnt = var("cnt", DType::Int32), u = var("u", DType::Int32);
+  Expr jl = var("jl", DType::Int32), al = var("al", vt);
+  Expr j = var("j", DType::Int32), a = var("a", vt), k = var("k", DType::Int32);
+  Expr acc = var("acc", vt, true);
+  auto cidx = [&](Expr row) { return add(mul(i64(row), i64(C.dim2)), i64(k)); };
+  auto each_col = [&](Stmt body) {
+    return for_loop(dv, i32(0), i32(D), i32(1),
+                    block({decl(k, add(mul(dv, i32(W)), lane)), if_then(lt(k, Bm.dim2), body)}),
+                    LoopKind::Serial, ParallelUnit::NotParallel,
+                    OutputRaceStrategy::IgnoreRaces, D);
+  };
+  Stmt gather = for_loop(
+      u, i32(0), cnt, i32(1),
+      block({decl(j, call("taco_hip_team_bcast", {jl, u, i32(W)}, DType::Int32)),
+             decl(a, call("taco_hip_team_bcast", {al, u, i32(W)}, vt)),
+             each_col(store(acc, dv, add(load(acc, dv),
+                                         mul(a, load(Bm.vals, add(mul(i64(j), i64(Bm.dim2)),
+                                                                  i64(k)))))))}));
+  Stmt chunks = for_loop(
+      zc, fA, zstop, i32(W),
+      block({decl(cnt, min(i32(W), sub(zstop, zc))),
+             decl(jl, ternary(lt(lane, cnt), load(A.crd, add(zc, lane)), i32(0))),
+             decl(al, ternary(lt(lane, cnt), load(A.vals, add(zc, lane)), zero(vt))),
+             gather}));
+  Stmt row_end = block({
+      each_col(block({if_then(eq(i, i_first),
+                              store_add(C.vals, cidx(i), load(acc, dv), true,
+                                        ParallelUnit::GPUThread),
+                              store(C.vals, cidx(i), load(acc, dv))),
+                      store(acc, dv, zero(vt))})),
+      assign(i, add(i, i32(1))), assign(d, add(d, lit(1, DType::Int64)))});
+  Stmt walk = while_loop(
+      lt(d, dend),
+      block({decl(rend, load(A.pos, add(i, i32(1)))),
+             decl(zstop, cast(DType::Int32, min(add(i64(fA), sub(dend, d)), i64(rend)))),
+             chunks,
+             assign(d, add(d, i64(sub(zstop, fA)))), assign(fA, zstop),
+             if_then(land(lt(d, dend), eq(fA, rend)), row_end)}));
+  Stmt team_body = block({
+      decl(d, mul(add(mul(i64(block_v), lit(teams, DType::Int64)), i64(warp)),
+                  lit(P, DType::Int64))),
+      if_then(lt(d, total),
+              block({decl(dend, min(add(d, lit(P, DType::Int64)), total)),
+                     decl(i, call("taco_hip_mergePathSearch", {A.pos, A.dim1, d}, DType::Int32)),
+                     decl(i_first, i), decl(fA, cast(DType::Int32, sub(d, i64(i)))),
+                     allocate(acc, i32(D), true), walk,
+                     if_then(lt(i, A.dim1),
+                             each_col(store_add(C.vals, cidx(i), load(acc, dv), true,
+                                                ParallelUnit::GPUThread)))}))});
+  return for_loop(
+      block_v, i32(0), nb, i32(1),
+      for_loop(warp, i32(0), i32(teams), i32(1),
+               for_loop(lane, i32(0), i32(W), i32(1), team_body, LoopKind::Serial,
+                        ParallelUnit::GPUThread, OutputRaceStrategy::Atomics),
+               LoopKind::Serial, ParallelUnit::GPUWarp),
+      LoopKind::Serial, ParallelUnit::GPUBlock);
+}
+
 std::vector<Stmt> lower_spmm(const taco_hip_gen_spec& s, DType vt) {
   Csr A = unpack_csr("A", vt);
   Dense Bm = unpack_dense("B", vt, 2), C = unpack_dense("C", vt, 2);
@@ -401,6 +489,7 @@ std::vector<Stmt> lower_spmm(const taco_hip_gen_spec& s, DType vt) {
   switch (s.family) {
     case TACO_HIP_GEN_DEFAULT: k = spmm_default(s, vt, A, Bm, C, pre); break;
     case TACO_HIP_GEN_NNZ_SPLIT: k = spmm_nnz_split(s, vt, A, Bm, C, pre); break;
+    case TACO_HIP_GEN_MERGE: k = spmm_merge(s, vt, A, Bm, C, pre); break;
     default: throw std::runtime_error("SpMM: unsupported schedule family");
   }
   pre.push_back(k);

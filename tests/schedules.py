@@ -77,3 +77,24 @@ def schedule_spmm_gpu(stmt, A, precomputed, i, j, k, NNZ_PER_WARP=8, BLOCK_SIZE=
             .parallelize(block, PU.GPUBlock, ORS.IgnoreRaces)
             .parallelize(warp, PU.GPUWarp, ORS.IgnoreRaces)
             .parallelize(thread, PU.GPUThread, ORS.Atomics))
+
+
+def schedule_spmm_merge_gpu(stmt, A, precomputed, i, j, k, ITEMS_PER_TEAM=256, BLOCK_SIZE=256,
+                            LANES=64, COLS_PER_LANE=2):
+    """The HIP backend's merge-path SpMM: scheduleSpMMGPU's loop structure
+    with the fused (i,j) space split directly (rows and entries together),
+    without pos()."""
+    f, f1, item = IndexVar("f"), IndexVar("f1"), IndexVar("item")
+    block, warp = IndexVar("block"), IndexVar("warp")
+    dvu, dv, thread = IndexVar("dense_val_unbounded"), IndexVar("dense_val"), IndexVar("thread")
+    per_tb = ITEMS_PER_TEAM * (BLOCK_SIZE // LANES)
+    return (stmt.reorder([i, j, k])
+            .fuse(i, j, f)
+            .split(f, block, f1, per_tb)
+            .split(f1, warp, item, ITEMS_PER_TEAM)
+            .split(k, dvu, thread, LANES)
+            .reorder([block, warp, thread, dvu, item])
+            .bound(dvu, dv, COLS_PER_LANE, BoundType.MaxExact)
+            .parallelize(block, PU.GPUBlock, ORS.IgnoreRaces)
+            .parallelize(warp, PU.GPUWarp, ORS.IgnoreRaces)
+            .parallelize(thread, PU.GPUThread, ORS.Atomics))

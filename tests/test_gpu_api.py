@@ -61,7 +61,8 @@ def test_spmv_gpu_reference_inputs(schedule):
     assert np.array_equal(y.to_numpy(), d["spmvgpu_y"])
 
 
-def test_spmm_gpu_reference_inputs():
+@pytest.mark.parametrize("schedule", ["spmm_gpu", "merge"])
+def test_spmm_gpu_reference_inputs(schedule):
     d = np.load(os.path.join(GOLD, "ref_tests.npz"))
     i, j, k = taco.IndexVar("i"), taco.IndexVar("j"), taco.IndexVar("k")
     A = csr_tensor("A", [102, 103], d["spmmgpu_pos"], d["spmmgpu_crd"], d["spmmgpu_vals"])
@@ -70,7 +71,8 @@ def test_spmm_gpu_reference_inputs():
     C = taco.Tensor("C", [102, 128], taco.dense_matrix)
     precomputed = A[i, j] * B[j, k]
     C[i, k] = precomputed
-    stmt = S.schedule_spmm_gpu(C.get_assignment().concretize(), A, precomputed, i, j, k)
+    sched = S.schedule_spmm_gpu if schedule == "spmm_gpu" else S.schedule_spmm_merge_gpu
+    stmt = sched(C.get_assignment().concretize(), A, precomputed, i, j, k)
     C.compile(stmt)
     C.assemble()
     C.compute()

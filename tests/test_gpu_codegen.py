@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 SPMV = ["spmv_default_f64", "spmv_rows_f64", "spmv_rows64_f64", "spmv_nnz_f64", "spmv_merge_f64"]
-SPMM = ["spmm_default_f64", "spmm_nnz_f64"]
+SPMM = ["spmm_default_f64", "spmm_nnz_f64", "spmm_merge_f64"]
 
 
 def mats():
