@@ -9,7 +9,11 @@ Two forms:
     summation order); a candidate whose result differs beyond the value
     tolerance is never selected.
   * ``tune_spmv`` / ``tune_spmm`` — one matrix, every schedule in the
-    candidate space timed with HIP events; returns the ranking.
+    candidate space timed with HIP events; returns the ranking.  The first
+    candidate (one thread per row, the sequential summation order) gives
+    the reference output; every other candidate's output is checked against
+    it before it is timed, and one that differs beyond the value tolerance
+    is ranked last with time inf.
 
 The candidate spaces are the schedule families the kernels implement; they
 are also the classes the learned selector predicts.
@@ -74,23 +78,48 @@ def _time(fn, reps, warmup=2):
     return e0.elapsed_time(e1) / reps
 
 
-def tune_spmv(a_desc, x_desc, y_desc, reps=5, candidates=None):
-    """Times every SpMV schedule on one matrix; returns [(name, ms)] sorted."""
-    L = _lib.lib()
-    out = []
-    for name, s in candidates or SPMV_CANDIDATES:
-        fn = lambda s=s: _lib.check(L.taco_hip_spmv_compute(y_desc.ptr, a_desc.ptr, x_desc.ptr, s))
+def accepts(out, ref, rtol):
+    """True when out matches ref to rtol of ref's largest magnitude."""
+    if ref.numel() == 0:
+        return True
+    scale = max(float(ref.abs().max().item()), 1e-30)
+    diff = (out.double() - ref.double()).abs().max().item()
+    return bool(diff <= rtol * scale)  # NaN fails
+
+
+def _tune(runs, result, reps, rtol):
+    import torch
+    out, ref = [], None
+    for name, fn in runs:
+        fn()
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = result.clone()
+        elif not accepts(result, ref, rtol):
+            out.append((name, float("inf")))
+            continue
         out.append((name, _time(fn, reps)))
     return sorted(out, key=lambda t: t[1])
 
 
-def tune_spmm(a_desc, b_desc, c_desc, K, reps=3, candidates=None):
+def tune_spmv(a_desc, x_desc, y_desc, reps=5, candidates=None, rtol=1e-10):
+    """Times every SpMV schedule on one matrix; returns [(name, ms)] sorted
+    (inf: output rejected).  y_desc must describe a device tensor."""
     L = _lib.lib()
-    out = []
-    for name, s in candidates or spmm_candidates(K):
-        fn = lambda s=s: _lib.check(L.taco_hip_spmm_compute(c_desc.ptr, a_desc.ptr, b_desc.ptr, s))
-        out.append((name, _time(fn, reps)))
-    return sorted(out, key=lambda t: t[1])
+    runs = [(name, lambda s=s: _lib.check(
+        L.taco_hip_spmv_compute(y_desc.ptr, a_desc.ptr, x_desc.ptr, s)))
+        for name, s in candidates or SPMV_CANDIDATES]
+    return _tune(runs, y_desc.keep[0], reps, rtol)
+
+
+def tune_spmm(a_desc, b_desc, c_desc, K, reps=3, candidates=None, rtol=1e-4):
+    """Times every SpMM schedule on one matrix; returns [(name, ms)] sorted
+    (inf: output rejected).  c_desc must describe a device tensor."""
+    L = _lib.lib()
+    runs = [(name, lambda s=s: _lib.check(
+        L.taco_hip_spmm_compute(c_desc.ptr, a_desc.ptr, b_desc.ptr, s)))
+        for name, s in candidates or spmm_candidates(K)]
+    return _tune(runs, c_desc.keep[0], reps, rtol)
 
 
 def tune_batch(wl, op, reps=3, rtol=1e-4, check=True):
