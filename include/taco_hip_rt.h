@@ -266,6 +266,54 @@ __device__ inline bool taco_hip_row_candidate(
   return false;
 }
 
+/* Short-row candidates in lanes, loaded cooperatively: lane t < (row length
+ * of A) reads A's entry t and its B row bounds, a wave prefix sum of the B
+ * row lengths places candidate q (encounter order) in lane q, which then
+ * reads its column.  Returns the row's candidate count when A's row has at
+ * most 64 entries (lanes hold candidates only if it is <= 64), or -1 when A's
+ * row is longer (nothing loaded).  Every lane of the wave must call it. */
+__device__ inline int64_t taco_hip_row_lanes(
+    const int32_t* apos, const int32_t* acrd, const int32_t* bpos,
+    const int32_t* bcrd, int32_t i, int32_t* col, int32_t* jA, int32_t* kB) {
+  const int lane = (int)__lane_id();
+  const int32_t a0 = apos[i], la = apos[i + 1] - a0;
+  if (la > 64) return -1;
+  int32_t b0 = 0, len = 0;
+  if (lane < la) {
+    const int32_t j = acrd[a0 + lane];
+    b0 = bpos[j];
+    len = bpos[j + 1] - b0;
+  }
+  /* B row lengths clamped to 65: the prefix sums stay in int32 and a total
+   * above 64 still reads as above 64. */
+  const int32_t lc = len < 65 ? len : 65;
+  int32_t inc = lc;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t u = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += u;
+  }
+  const int32_t total = __shfl(inc, 63, 64);
+  if (total > 64) {
+    int64_t n = len;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+    return n;
+  }
+  /* Entry t holding candidate `lane`: the number of entries s < la - 1 whose
+   * inclusive sum is <= lane. */
+  int t = 0;
+  for (int s2 = 0; s2 + 1 < la; ++s2) t += __shfl(inc, s2, 64) <= lane ? 1 : 0;
+  const int32_t excl = __shfl(inc - lc, t, 64);
+  const int32_t bt = __shfl(b0, t, 64);
+  if (lane < total) {
+    *jA = a0 + t;
+    *kB = bt + (lane - excl);
+    *col = bcrd[*kB];
+  }
+  return total;
+}
+
 __device__ inline void taco_hip_wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -338,10 +386,14 @@ __device__ inline int32_t taco_hip_spgemm_row_symbolic(
     int32_t* status) {
   __shared__ TacoHipRowSet s_sets[TACO_HIP_ROW_WAVES];
   const int lane = (int)__lane_id();
-  const int64_t n = taco_hip_row_candidates(apos, acrd, bpos, i);
+  int32_t col = INT_MAX, jA = 0, kB = 0;
+  int64_t n = taco_hip_row_lanes(apos, acrd, bpos, bcrd, i, &col, &jA, &kB);
+  if (n < 0) {
+    n = taco_hip_row_candidates(apos, acrd, bpos, i);
+    if (n <= 64 && lane < n)
+      taco_hip_row_candidate(apos, acrd, bpos, bcrd, i, lane, &col, &jA, &kB);
+  }
   if (n <= 64) {
-    int32_t col = INT_MAX, jA = 0, kB = 0;
-    if (lane < n) taco_hip_row_candidate(apos, acrd, bpos, bcrd, i, lane, &col, &jA, &kB);
     const int32_t s = taco_hip_wave_sort(col);
     const int32_t prev = __shfl_up(s, 1, 64);
     const bool head = s != INT_MAX && (lane == 0 || prev != s);
@@ -372,15 +424,17 @@ __device__ inline void taco_hip_spgemm_row_numeric(
     const int32_t* cpos, const int32_t* ccrd, double* cvals) {
   __shared__ double s_acc[TACO_HIP_ROW_WAVES][TACO_HIP_ROW_CAP];
   const int lane = (int)__lane_id();
-  const int64_t n = taco_hip_row_candidates(apos, acrd, bpos, i);
   const int32_t c0 = cpos[i], nc = cpos[i + 1] - c0;
-  if (n <= 64) {
-    int32_t col = INT_MAX, jA = 0, kB = 0;
-    double prod = 0;
-    if (lane < n) {
+  int32_t col = INT_MAX, jA = 0, kB = 0;
+  int64_t n = taco_hip_row_lanes(apos, acrd, bpos, bcrd, i, &col, &jA, &kB);
+  if (n < 0) {
+    n = taco_hip_row_candidates(apos, acrd, bpos, i);
+    if (n <= 64 && lane < n)
       taco_hip_row_candidate(apos, acrd, bpos, bcrd, i, lane, &col, &jA, &kB);
-      prod = avals[jA] * bvals[kB];
-    }
+  }
+  if (n <= 64) {
+    double prod = 0;
+    if (lane < n) prod = avals[jA] * bvals[kB];
     /* Key = column, then encounter index (which also names the lane). */
     uint64_t key = ((uint64_t)(uint32_t)col << 32) | (uint32_t)lane;
     key = taco_hip_wave_sort(key);

@@ -195,6 +195,14 @@ def spgemm_cases():
         W.csr_fixed(1200, 2000, 3, 24)
     yield "empty", W.Csr(4, 3, np.zeros(5, np.int32), np.zeros(0, np.int32),
                          np.zeros(0)), W.csr_fixed(3, 6, 2, 25)
+    # A rows longer than 64 entries over mostly empty B rows: few candidates
+    yield "long_a_empty_b", W.csr_fixed(600, 2000, 80, 26), W.csr_from_counts(
+        2000, 900, np.where(np.arange(2000) % 50 == 0, 3, 0).astype(np.int32), 27)
+    # short A rows over B rows that are often empty and sometimes longer than 64
+    counts = np.random.default_rng(29).integers(0, 12, 2500).astype(np.int32)
+    counts[np.random.default_rng(30).random(2500) < 0.4] = 0
+    counts[::97] = 70
+    yield "gappy_b", W.csr_fixed(3000, 2500, 6, 28), W.csr_from_counts(2500, 3000, counts, 31)
 
 
 @pytest.mark.parametrize("name,A,B", list(spgemm_cases()), ids=lambda v: v if isinstance(v, str) else "")
@@ -206,7 +214,7 @@ def test_spgemm(name, A, B):
     ex, cond = oracle.exact_spgemm_values(A, B, opos, ocrd)
     rep = oracle.error_report(vals, ex, cond, TOL["f64"])
     assert rep["ok"], rep
-    if name == "fixed8":
+    if name in ("fixed8", "long_a_empty_b"):
         # Short rows sum duplicates sequentially in taco's encounter order,
         # with separately rounded products: equal to taco's values bit for bit.
         assert np.array_equal(vals, ovals)
