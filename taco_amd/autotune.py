@@ -7,7 +7,10 @@ Two forms:
     candidate).  Each candidate's whole-batch output is first checked
     against the reference candidate (one lane per row, the sequential
     summation order); a candidate whose result differs beyond the value
-    tolerance is never selected.
+    tolerance is never selected.  Instances alone do not see the L2 and
+    queue contention of the whole launch, so the per-instance choice is
+    then raced, as whole-batch launches, against every accepted candidate
+    applied to all instances; the fastest whole launch wins.
   * ``tune_spmv`` / ``tune_spmm`` — one matrix, every schedule in the
     candidate space timed with HIP events; returns the ranking.  The first
     candidate (one thread per row, the sequential summation order) gives
@@ -125,8 +128,10 @@ def tune_spmm(a_desc, b_desc, c_desc, K, reps=3, candidates=None, rtol=1e-4):
 def tune_batch(wl, op, reps=3, rtol=1e-4, check=True):
     """Exhaustive per-instance schedule search for a batched launch.
 
-    Returns (best lanes per instance, times[candidate, instance] in ns,
-    candidate lane counts, candidates rejected by the output check)."""
+    Returns (chosen lanes per instance, times[candidate, instance] in ns,
+    candidate lane counts, candidates rejected by the output check).  The
+    whole-batch times of the raced schedules are in tune_batch.last_race
+    ({name: ms})."""
     import torch
     cands = list(BT.SPMV_LANES if op == 0 else BT.SPMM_LANES)
     ref_lanes = cands[0]
@@ -149,5 +154,15 @@ def tune_batch(wl, op, reps=3, rtol=1e-4, check=True):
                 rejected.append(lanes)
                 times[ci] = np.inf
     best = np.array(cands, np.int32)[np.argmin(times, axis=0)]
-    b.set_schedules(best)
-    return best, times, cands, rejected
+    race = {"per_instance": best}
+    for lanes in cands:
+        if lanes not in rejected:
+            race[f"all_{lanes}"] = np.full(wl.ninst, lanes, np.int32)
+    race_ms = {}
+    for name, sched in race.items():
+        b.set_schedules(sched)
+        race_ms[name] = _time(b.run, reps, warmup=1)
+    tune_batch.last_race = race_ms
+    chosen = race[min(race_ms, key=race_ms.get)]
+    b.set_schedules(chosen)
+    return chosen, times, cands, rejected

@@ -188,7 +188,7 @@ class Batch:
         return {"instances": out[0], "tasks": out[1], "workgroups": out[2]}
 
     def __del__(self):
-        if getattr(self, "_p", None) and _lib._lib is not None:
+        if getattr(self, "_p", None) and _lib is not None and _lib._lib is not None:
             _lib._lib.taco_hip_batch_destroy(self._p)
             self._p = None
 

@@ -20,11 +20,19 @@
 //         by entries instead of rows; suits short rows)
 //   SpMM: L lanes per row along k (K / L columns per lane), L in {8,16,32,64};
 //         when K = 4L each lane owns 4 adjacent columns (16-byte accesses)
-// A launch may cover a contiguous range of instances (their tasks are
-// contiguous), which the autotuner times one instance at a time with HIP
-// events.  The queue counter is reset by a one-thread kernel, so the pair
-// (reset, batch) can be captured into a hipGraph and replayed.
+// XCD affinity: a whole-batch launch gives each of the 8 XCDs its own task
+// queue.  The host assigns instances to XCDs by work (largest first, to the
+// least loaded), so an XCD's L2 holds only the x / B rows of its own
+// instances.  Workgroup w runs on XCD w % 8 (round-robin dispatch); it drains
+// its XCD's queue, then takes tasks from the other queues in turn, so the
+// launch ends balanced.  Which workgroup runs a task never changes its
+// result.
+// A launch may also cover a contiguous range of instances (their tasks are
+// contiguous) through one queue; the autotuner times one instance at a time
+// that way with HIP events.  The queue counters are reset by a small kernel,
+// so the pair (reset, batch) can be captured into a hipGraph and replayed.
 
+#include <algorithm>
 #include <vector>
 
 #include "common.h"
@@ -46,22 +54,57 @@ constexpr int kBatchThreads = 256;
 // Products staged in LDS per chunk by the stream schedule (16 KB).
 constexpr int kStreamChunk = 4096;
 
-__global__ void k_reset_queue(uint32_t* q) { q[0] = 0; }
+constexpr int kXcds = 8;
+// Queue counters sit 128 bytes apart.
+constexpr int kQStride = 32;
+
+__global__ void k_reset_queue(uint32_t* q, int32_t nq) {
+  if ((int32_t)threadIdx.x < nq) q[threadIdx.x * kQStride] = 0;
+}
+
+// The queues of one launch: nq == 1 serves task slots [lo, hi) directly;
+// nq > 1 serves slots [bound[k], bound[k+1]) of queue k, mapped to tasks by
+// order[].
+struct Queues {
+  int32_t nq, lo, hi;
+  const int32_t* bound;
+  const int32_t* order;
+  uint32_t* counter;
+};
+
+// Calls body(task index) for every task the workgroup takes.  All threads of
+// the workgroup call it together; every counter ends past its queue's end,
+// so every workgroup leaves the loop.
+template <typename Body>
+__device__ __forceinline__ void for_each_task(const Queues& qs, int32_t* s_slot, Body body) {
+  const int32_t home = qs.nq > 1 ? (int32_t)(blockIdx.x % (uint32_t)qs.nq) : 0;
+  for (int32_t k = 0; k < qs.nq; ++k) {
+    const int32_t qi = home + k < qs.nq ? home + k : home + k - qs.nq;
+    const int32_t lo = qs.nq > 1 ? qs.bound[qi] : qs.lo;
+    const int32_t hi = qs.nq > 1 ? qs.bound[qi + 1] : qs.hi;
+    for (;;) {
+      if (threadIdx.x == 0) {
+        const uint32_t c = atomicAdd(qs.counter + qi * kQStride, 1u);
+        *s_slot = c < (uint32_t)(hi - lo) ? lo + (int32_t)c : -1;
+      }
+      __syncthreads();
+      const int32_t slot = *s_slot;
+      __syncthreads();
+      if (slot < 0) break;
+      body(qs.order ? qs.order[slot] : slot);
+    }
+  }
+}
 
 __global__ __launch_bounds__(kBatchThreads) void k_batch_spmv(
-    int32_t ntasks, const Task* __restrict__ tasks, const Inst* __restrict__ insts,
+    Queues qs, const Task* __restrict__ tasks, const Inst* __restrict__ insts,
     const int32_t* __restrict__ pos, const int32_t* __restrict__ crd,
     const float* __restrict__ vals, const float* __restrict__ x,
-    float* __restrict__ y, uint32_t* __restrict__ queue) {
-  __shared__ int32_t s_task;
+    float* __restrict__ y) {
+  __shared__ int32_t s_slot;
   __shared__ float s_prod[kStreamChunk];
   const int tid = threadIdx.x;
-  for (;;) {
-    if (tid == 0) s_task = (int32_t)atomicAdd(queue, 1u);
-    __syncthreads();
-    const int32_t t = s_task;
-    __syncthreads();
-    if (t >= ntasks) break;  // every thread of the workgroup sees the same t
+  for_each_task(qs, &s_slot, [&](int32_t t) {
     const Task tk = tasks[t];
     const Inst in = insts[tk.inst];
     const int G = in.sched;  // lanes per row (power of two <= 64), 0 = stream
@@ -92,7 +135,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmv(
         }
         if (r < re_) iy[r] = acc;
       }
-      continue;
+      return;
     }
     const int lane = tid & (G - 1);
     const int rows_per_pass = kBatchThreads / G;
@@ -104,23 +147,17 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmv(
       for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
       if (lane == 0) iy[r] = acc;
     }
-  }
+  });
 }
 
 __global__ __launch_bounds__(kBatchThreads) void k_batch_spmm(
-    int32_t ntasks, int32_t K, const Task* __restrict__ tasks,
+    Queues qs, int32_t K, const Task* __restrict__ tasks,
     const Inst* __restrict__ insts, const int32_t* __restrict__ pos,
     const int32_t* __restrict__ crd, const float* __restrict__ vals,
-    const float* __restrict__ B, float* __restrict__ C,
-    uint32_t* __restrict__ queue) {
-  __shared__ int32_t s_task;
+    const float* __restrict__ B, float* __restrict__ C) {
+  __shared__ int32_t s_slot;
   const int tid = threadIdx.x;
-  for (;;) {
-    if (tid == 0) s_task = (int32_t)atomicAdd(queue, 1u);
-    __syncthreads();
-    const int32_t t = s_task;
-    __syncthreads();
-    if (t >= ntasks) break;
+  for_each_task(qs, &s_slot, [&](int32_t t) {
     const Task tk = tasks[t];
     const Inst in = insts[tk.inst];
     const int L = in.sched;  // lanes per row along k
@@ -159,7 +196,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmm(
         }
         *(f32x4*)(iC + (int64_t)r * K + 4 * lane) = acc;
       }
-      continue;
+      return;
     }
     for (int32_t r = tk.r0 + tid / L; r < tk.r1; r += rows_per_pass) {
       float acc[4] = {0.f, 0.f, 0.f, 0.f};  // K <= 4 * L
@@ -179,7 +216,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmm(
         if (k < K) iC[(int64_t)r * K + k] = acc[c];
       }
     }
-  }
+  });
 }
 
 // 32-bit columns of a generated f32 value stream: U[lo, hi) from the same
@@ -211,7 +248,9 @@ struct taco_hip_batch {
   int op = 0;  // 0 SpMV, 1 SpMM
   Inst* insts = nullptr;
   Task* tasks = nullptr;
-  uint32_t* queue = nullptr;
+  uint32_t* queue = nullptr;   // kXcds counters, kQStride apart
+  int32_t* order = nullptr;    // task indices grouped by XCD queue
+  int32_t* qbound = nullptr;   // kXcds + 1 slot offsets into order
   const int32_t* pos = nullptr;
   const int32_t* crd = nullptr;
   const float* vals = nullptr;
@@ -229,24 +268,59 @@ void batch_free(taco_hip_batch* b) {
   if (!b) return;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (void* q : {(void*)b->insts, (void*)b->tasks, (void*)b->queue})
+  for (void* q : {(void*)b->insts, (void*)b->tasks, (void*)b->queue, (void*)b->order,
+                  (void*)b->qbound})
     if (q) (void)hipFree(q);
   delete b;
 }
 
-int launch_batch(taco_hip_batch* b, hipStream_t st, int32_t t0, int32_t nt) {
+// whole: every task through the per-XCD queues; otherwise tasks [t0, t0+nt)
+// through one queue.
+int launch_batch(taco_hip_batch* b, hipStream_t st, int32_t t0, int32_t nt, bool whole) {
   if (nt <= 0) return 0;
-  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, st, b->queue);
+  Queues qs{1, t0, t0 + nt, nullptr, nullptr, b->queue};
+  if (whole) qs = Queues{kXcds, 0, 0, b->qbound, b->order, b->queue};
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, st, b->queue, qs.nq);
   const unsigned grid = (unsigned)(b->grid < nt ? b->grid : nt);
-  const Task* tasks = b->tasks + t0;
   if (b->op == 0)
-    hipLaunchKernelGGL(k_batch_spmv, dim3(grid), dim3(kBatchThreads), 0, st, nt, tasks,
-                       b->insts, b->pos, b->crd, b->vals, b->in, b->out, b->queue);
+    hipLaunchKernelGGL(k_batch_spmv, dim3(grid), dim3(kBatchThreads), 0, st, qs, b->tasks,
+                       b->insts, b->pos, b->crd, b->vals, b->in, b->out);
   else
-    hipLaunchKernelGGL(k_batch_spmm, dim3(grid), dim3(kBatchThreads), 0, st, nt, b->K,
-                       tasks, b->insts, b->pos, b->crd, b->vals, b->in, b->out, b->queue);
+    hipLaunchKernelGGL(k_batch_spmm, dim3(grid), dim3(kBatchThreads), 0, st, qs, b->K,
+                       b->tasks, b->insts, b->pos, b->crd, b->vals, b->in, b->out);
   TH_LAUNCH_CHECK("batched kernel");
   return 0;
+}
+
+// Instances to XCD queues, largest first to the least loaded (work ~ rows +
+// entries); returns the task order grouped by queue and the queue bounds.
+void xcd_order(const std::vector<Inst>& insts, const int64_t* nnz,
+               const std::vector<int32_t>& task_begin, std::vector<int32_t>* order,
+               std::vector<int32_t>* bound) {
+  const int32_t ninst = (int32_t)insts.size();
+  std::vector<int32_t> by_work((size_t)ninst);
+  for (int32_t i = 0; i < ninst; ++i) by_work[(size_t)i] = i;
+  auto work = [&](int32_t i) { return (int64_t)insts[(size_t)i].m + nnz[i]; };
+  std::stable_sort(by_work.begin(), by_work.end(),
+                   [&](int32_t a, int32_t c) { return work(a) > work(c); });
+  std::vector<int64_t> load(kXcds, 0);
+  std::vector<int32_t> home((size_t)ninst, 0);
+  for (int32_t i : by_work) {
+    int x = 0;
+    for (int k = 1; k < kXcds; ++k)
+      if (load[(size_t)k] < load[(size_t)x]) x = k;
+    home[(size_t)i] = x;
+    load[(size_t)x] += work(i);
+  }
+  order->clear();
+  bound->assign(kXcds + 1, 0);
+  for (int x = 0; x < kXcds; ++x) {
+    for (int32_t i = 0; i < ninst; ++i)
+      if (home[(size_t)i] == x)
+        for (int32_t t = task_begin[(size_t)i]; t < task_begin[(size_t)i + 1]; ++t)
+          order->push_back(t);
+    (*bound)[(size_t)x + 1] = (int32_t)order->size();
+  }
 }
 
 bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
@@ -316,10 +390,14 @@ int taco_hip_batch_create(int32_t op, int32_t ninst, const int32_t* m,
     return fail("batch: too many tasks");
   }
   b->ntasks = (int32_t)tasks.size();
+  std::vector<int32_t> order, bound;
+  xcd_order(b->host_insts, nnz, b->task_begin, &order, &bound);
   hipStream_t st = current_stream();
   if (hipMalloc((void**)&b->insts, sizeof(Inst) * (ninst > 0 ? ninst : 1)) ||
       hipMalloc((void**)&b->tasks, sizeof(Task) * (tasks.empty() ? 1 : tasks.size())) ||
-      hipMalloc((void**)&b->queue, 64)) {
+      hipMalloc((void**)&b->queue, sizeof(uint32_t) * kQStride * kXcds) ||
+      hipMalloc((void**)&b->order, sizeof(int32_t) * (order.empty() ? 1 : order.size())) ||
+      hipMalloc((void**)&b->qbound, sizeof(int32_t) * (kXcds + 1))) {
     batch_free(b);
     return fail("batch: device allocation failed");
   }
@@ -327,6 +405,10 @@ int taco_hip_batch_create(int32_t op, int32_t ninst, const int32_t* m,
                                    hipMemcpyHostToDevice, st)) ||
       (!tasks.empty() && hipMemcpyAsync(b->tasks, tasks.data(), sizeof(Task) * tasks.size(),
                                         hipMemcpyHostToDevice, st)) ||
+      (!order.empty() && hipMemcpyAsync(b->order, order.data(), sizeof(int32_t) * order.size(),
+                                        hipMemcpyHostToDevice, st)) ||
+      hipMemcpyAsync(b->qbound, bound.data(), sizeof(int32_t) * (kXcds + 1),
+                     hipMemcpyHostToDevice, st) ||
       hipStreamSynchronize(st)) {
     batch_free(b);
     return fail("batch: upload failed");
@@ -357,7 +439,7 @@ int taco_hip_batch_set_schedules(taco_hip_batch* b, const int32_t* sched) {
 
 int taco_hip_batch_run(taco_hip_batch* b) {
   TH_REQUIRE(b, "batch: null");
-  if (int rc = launch_batch(b, current_stream(), 0, b->ntasks)) return rc;
+  if (int rc = launch_batch(b, current_stream(), 0, b->ntasks, true)) return rc;
   return finish("taco_hip_batch_run");
 }
 
@@ -365,7 +447,7 @@ int taco_hip_batch_run(taco_hip_batch* b) {
 int taco_hip_batch_run_range(taco_hip_batch* b, int32_t i0, int32_t i1) {
   TH_REQUIRE(b && i0 >= 0 && i0 <= i1 && i1 <= b->ninst, "batch: bad instance range");
   const int32_t t0 = b->task_begin[(size_t)i0], t1 = b->task_begin[(size_t)i1];
-  if (int rc = launch_batch(b, current_stream(), t0, t1 - t0)) return rc;
+  if (int rc = launch_batch(b, current_stream(), t0, t1 - t0, false)) return rc;
   return finish("taco_hip_batch_run_range");
 }
 
@@ -391,7 +473,7 @@ int taco_hip_batch_time_instances(taco_hip_batch* b, int32_t reps, double* times
       break;
     }
     const int32_t t0 = b->task_begin[(size_t)i], t1 = b->task_begin[(size_t)i + 1];
-    for (int32_t r = 0; rc == 0 && r < reps; ++r) rc = launch_batch(b, st, t0, t1 - t0);
+    for (int32_t r = 0; rc == 0 && r < reps; ++r) rc = launch_batch(b, st, t0, t1 - t0, false);
   }
   if (rc == 0 && hipEventRecord(ev[(size_t)b->ninst], st) != hipSuccess)
     rc = fail("batch: hipEventRecord failed");
@@ -420,7 +502,7 @@ int taco_hip_batch_capture(taco_hip_batch* b) {
   if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
     rc = fail("batch: begin capture failed");
   } else {
-    rc = launch_batch(b, cap, 0, b->ntasks);
+    rc = launch_batch(b, cap, 0, b->ntasks, true);
     hipGraph_t g = nullptr;
     if (hipStreamEndCapture(cap, &g) != hipSuccess && !rc) rc = fail("batch: end capture failed");
     if (!rc) {
