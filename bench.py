@@ -437,11 +437,20 @@ def bench_batched(args, op):
     log(f"batched: {wl.ninst} instances, {wl.total_rows} rows, {wl.total_nnz} entries, "
         f"generated in {gen_s:.1f}s")
     t0 = time.perf_counter()
-    best, times, cands, rejected = autotune.tune_batch(wl, op, reps=3)
+    if args.batched_sched is None:
+        best, times, cands, rejected = autotune.tune_batch(wl, op, reps=3)
+        race = autotune.tune_batch.last_race
+        runs = (("default", BT.default_schedule(op, wl.ninst)), ("autotuned", best))
+    else:
+        # one schedule for every instance, no autotuning (for profiling the
+        # whole-batch kernel alone)
+        best = np.full(wl.ninst, args.batched_sched, np.int32)
+        rejected, race = [], None
+        runs = (("autotuned", best),)
     tune_s = time.perf_counter() - t0
     timer = Timer(torch)
     res = {}
-    for name, sched in (("default", BT.default_schedule(op, wl.ninst)), ("autotuned", best)):
+    for name, sched in runs:
         b = BT.Batch(wl, op, sched)
         b.capture()
         ms, wall = timer.run(b.launch_graph, args.steps, args.warmup)
@@ -459,14 +468,18 @@ def bench_batched(args, op):
         "dtype": "f32", "data": "synthetic (seed-generated)",
         "config": {"workload": f"batched_{'spmv' if op == 0 else 'spmm_k32'}_4096",
                    "instances": wl.ninst, "rows": wl.total_rows, "nnz": wl.total_nnz,
-                   "schedule": "lanes per row from the exhaustive autotuner (per instance, "
-                               "raced against uniform schedules on the whole batch)"},
+                   "schedule": ("lanes per row from the exhaustive autotuner (per instance, "
+                                "raced against uniform schedules on the whole batch)"
+                                if args.batched_sched is None else
+                                f"{args.batched_sched} lanes per row for every instance "
+                                "(0 = stream), no autotuning")},
         "gflops": flops / (ms * 1e-3) / 1e9, "nnz_per_s": wl.total_nnz / (ms * 1e-3),
-        "default_schedule": {"ms": res["default"]["ms"],
-                             "speedup_vs_default": res["default"]["ms"] / ms},
+        "default_schedule": ({"ms": res["default"]["ms"],
+                              "speedup_vs_default": res["default"]["ms"] / ms}
+                             if "default" in res else None),
         "autotune": {"seconds": tune_s, "rejected": rejected,
                      "chosen_lanes_histogram": {int(l): int((best == l).sum()) for l in lanes},
-                     "whole_batch_race_ms": autotune.tune_batch.last_race},
+                     "whole_batch_race_ms": race},
         "generate_seconds": gen_s,
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS, "traffic": None,
@@ -626,6 +639,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--instances", type=int, default=4096, help="batched workloads")
+    ap.add_argument("--batched-sched", type=int, default=None,
+                    help="batched workloads: one lane count for every instance, no autotuning")
     ap.add_argument("--problems", type=int, default=200, help="selector held-out problems")
     args = ap.parse_args()
     if args.gpus != 1:
