@@ -572,7 +572,8 @@ def bench_codegen(args):
     timer = Timer(torch)
     rows = {}
     for name in ("spmv_default_f64", "spmv_rows_f64", "spmv_rows64_f64", "spmv_nnz_f64",
-                 "spmv_merge_f64", "spmm_default_f64", "spmm_nnz_f64", "spmm_merge_f64"):
+                 "spmv_merge_f64", "spmv_block_f64", "spmm_default_f64", "spmm_nnz_f64",
+                 "spmm_merge_f64"):
         k = codegen.Kernel(codegen.standard(name))
         descs = (y, a, xd) if name.startswith("spmv") else (c, a, b)
         reps = 3 if "default" in name else args.steps

@@ -314,7 +314,12 @@ typedef enum taco_hip_gen_family {
    * SpMM: teams of warp_size lanes along k with `bound` columns per lane,
    * nnz_per_thread merge items per team, entries loaded by the team and
    * broadcast.  Rows shared between threads/teams are added atomically. */
-  TACO_HIP_GEN_MERGE = 4
+  TACO_HIP_GEN_MERGE = 4,
+  /* SpMV, one workgroup per row: pos(j) split over the workgroup's threads
+   * with a Temporary and no warp level, which taco turns into a
+   * GPUBlockReduction consumer (transformations.cpp:1049-1056; printed at
+   * codegen_cuda.cpp:875-890).  Reduced by taco_hip_block_sum. */
+  TACO_HIP_GEN_BLOCK_ROWS = 5
 } taco_hip_gen_family;
 
 typedef struct taco_hip_gen_spec {

@@ -137,6 +137,27 @@ __device__ inline T taco_hip_group_sum(T v) {
   return v;
 }
 
+/* Sum over the workgroup's B threads (GPUBlockReduction,
+ * codegen_cuda.cpp:875-890, where thread 0 adds the per-thread temporaries
+ * serially after a barrier).  Every thread of the workgroup calls it and gets
+ * the total: wave sums by a shuffle tree, then the per-wave sums through LDS,
+ * added in wave order, so the result does not depend on timing. */
+template <int B, typename T>
+__device__ inline T taco_hip_block_sum(T v) {
+  static_assert(B > 0 && B <= 1024 && (B <= 64 ? (B & (B - 1)) == 0 : B % 64 == 0),
+                "block reduction: B must be a power of two <= 64 or a multiple of 64 <= 1024");
+  if (B <= 64) return taco_hip_group_sum<B>(v);
+  __shared__ T s_wave[B / 64 > 0 ? B / 64 : 1];
+  v = taco_hip_group_sum<64>(v);
+  __syncthreads(); /* earlier calls have read s_wave */
+  if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x / 64] = v;
+  __syncthreads();
+  T t = s_wave[0];
+#pragma unroll
+  for (int w = 1; w < B / 64; ++w) t += s_wave[w];
+  return t;
+}
+
 __device__ inline void taco_hip_atomicAdd(double* a, int64_t i, double v) {
   unsafeAtomicAdd(a + i, v);
 }

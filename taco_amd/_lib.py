@@ -151,7 +151,7 @@ class GenSpec(Structure):
 
 
 OP_SPMV, OP_SPMM, OP_SPGEMM = range(3)
-GEN_DEFAULT, GEN_WARP_ROWS, GEN_NNZ_SPLIT, GEN_TWO_PHASE, GEN_MERGE = range(5)
+GEN_DEFAULT, GEN_WARP_ROWS, GEN_NNZ_SPLIT, GEN_TWO_PHASE, GEN_MERGE, GEN_BLOCK_ROWS = range(6)
 
 
 def generate(spec, what=0):

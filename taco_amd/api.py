@@ -265,6 +265,17 @@ def _recognise(stmt, kind, a):
         if ok:
             per_thread = s2[4]
             return (_lib.GEN_MERGE, s1[4] // per_thread, per_thread, 0, 0, 0)
+    if kind == "spmv" and ops == ["pos", "split", "parallelize", "parallelize"]:
+        # one workgroup per row; the thread Temporary without a GPUWarp
+        # parent reduces with GPUBlockReduction (transformations.cpp:1049-1056)
+        pos = next(c for c in cmds if c[0] == "pos")
+        s1 = sp[0]
+        ok = (pos[1] is j and s1[1] is pos[2]
+              and units.get(i, (None,))[0] == ParallelUnit.GPUBlock
+              and units.get(s1[3], (None, None)) == (ParallelUnit.GPUThread,
+                                                    OutputRaceStrategy.Temporary))
+        if ok:
+            return (_lib.GEN_BLOCK_ROWS, s1[4], 0, 0, 0, 0)
     if kind == "spmv" and ops == ["split", "split", "pos", "split", "parallelize",
                                   "parallelize", "parallelize"]:
         pos = next(c for c in cmds if c[0] == "pos")

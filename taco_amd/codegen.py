@@ -9,13 +9,14 @@ from . import _lib
 
 # Named schedules: the reference's GPU schedules at their default parameters
 # (test/tests-scheduling-eval.cpp), the f32 forms, and the HIP backend's
-# merge-path SpMV and SpMM.
+# merge-path SpMV and SpMM, and the block-per-row SpMV (GPUBlockReduction).
 STANDARD_SPECS = {
     "spmv_default_f64": (_lib.OP_SPMV, _lib.GEN_DEFAULT, 64, 256, 0, 0, 0, 0),
     "spmv_rows_f64": (_lib.OP_SPMV, _lib.GEN_WARP_ROWS, 64, 256, 0, 32, 1, 0),
     "spmv_rows64_f64": (_lib.OP_SPMV, _lib.GEN_WARP_ROWS, 64, 256, 0, 64, 1, 0),
     "spmv_nnz_f64": (_lib.OP_SPMV, _lib.GEN_NNZ_SPLIT, 64, 256, 8, 32, 0, 0),
     "spmv_merge_f64": (_lib.OP_SPMV, _lib.GEN_MERGE, 64, 256, 8, 0, 0, 0),
+    "spmv_block_f64": (_lib.OP_SPMV, _lib.GEN_BLOCK_ROWS, 64, 256, 0, 0, 0, 0),
     "spmv_default_f32": (_lib.OP_SPMV, _lib.GEN_DEFAULT, 32, 256, 0, 0, 0, 0),
     "spmv_nnz_f32": (_lib.OP_SPMV, _lib.GEN_NNZ_SPLIT, 32, 256, 8, 32, 0, 0),
     "spmm_default_f64": (_lib.OP_SPMM, _lib.GEN_DEFAULT, 64, 256, 0, 0, 0, 0),

@@ -150,6 +150,35 @@ Stmt spmv_warp_rows(const taco_hip_gen_spec& s, DType vt, const Csr& A,
                   LoopKind::Serial, ParallelUnit::GPUBlock);
 }
 
+// One workgroup per row: parallelize(i, GPUBlock), pos(j, jpos, A(i,j)),
+// split(jpos, jpos0, thread, B), parallelize(thread, GPUThread, Temporary).
+// With no GPUWarp parent the Temporary's consumer is a GPUBlockReduction
+// (transformations.cpp:1049-1056).
+Stmt spmv_block_rows(const taco_hip_gen_spec& s, DType vt, const Csr& A,
+                     const Dense& x, const Dense& y) {
+  const int B = s.block_size > 0 ? s.block_size : 256;
+  if (B > 1024 || (B <= 64 ? (B & (B - 1)) != 0 : B % 64 != 0))
+    throw std::runtime_error(
+        "BLOCK_ROWS: block size must be a power of two <= 64 or a multiple of 64 <= 1024");
+  Expr i = var("i", DType::Int32), thread = var("thread", DType::Int32);
+  Expr jpos0 = var("jpos0", DType::Int32), jposA = var("jposA", DType::Int32);
+  Expr j = var("j", DType::Int32), t = var("tthread_val", vt);
+  Expr row_len = sub(load(A.pos, add(i, i32(1))), load(A.pos, i));
+  Stmt producer = for_loop(
+      thread, i32(0), i32(B), i32(1),
+      for_loop(jpos0, i32(0), ceil_div(row_len, B), i32(1),
+               block({decl(jposA, add(load(A.pos, i), add(mul(jpos0, i32(B)), thread))),
+                      if_then(lt(jposA, load(A.pos, add(i, i32(1)))),
+                              block({decl(j, load(A.crd, jposA)),
+                                     assign(t, add(t, mul(load(A.vals, jposA),
+                                                          load(x.vals, j))))}))})),
+      LoopKind::Serial, ParallelUnit::GPUThread, OutputRaceStrategy::Temporary);
+  Stmt consumer = for_loop(thread, i32(0), i32(B), i32(1), store(y.vals, i, t),
+                           LoopKind::Serial, ParallelUnit::GPUBlockReduction);
+  return for_loop(i, i32(0), A.dim1, i32(1), block({decl(t, zero(vt)), producer, consumer}),
+                  LoopKind::Serial, ParallelUnit::GPUBlock);
+}
+
 // Entry count of A read on the host (taco reads A2_pos[A1_dimension]).
 Stmt nnz_decl(const Csr& A, Expr nnz) {
   return decl(nnz, load(A.pos, A.dim1));
@@ -291,6 +320,7 @@ std::vector<Stmt> lower_spmv(const taco_hip_gen_spec& s, DType vt) {
     case TACO_HIP_GEN_WARP_ROWS: k = spmv_warp_rows(s, vt, A, x, y); break;
     case TACO_HIP_GEN_NNZ_SPLIT: k = spmv_nnz_split(s, vt, A, x, y, pre); break;
     case TACO_HIP_GEN_MERGE: k = spmv_merge(s, vt, A, x, y, pre); break;
+    case TACO_HIP_GEN_BLOCK_ROWS: k = spmv_block_rows(s, vt, A, x, y); break;
     default: throw std::runtime_error("SpMV: unsupported schedule family");
   }
   pre.push_back(k);

@@ -98,3 +98,14 @@ def schedule_spmm_merge_gpu(stmt, A, precomputed, i, j, k, ITEMS_PER_TEAM=256, B
             .parallelize(block, PU.GPUBlock, ORS.IgnoreRaces)
             .parallelize(warp, PU.GPUWarp, ORS.IgnoreRaces)
             .parallelize(thread, PU.GPUThread, ORS.Atomics))
+
+
+def schedule_spmv_block_rows_gpu(stmt, A, precomputed, i, j, BLOCK_SIZE=256):
+    """One workgroup per row: the row's positions split over the workgroup's
+    threads with a Temporary and no warp level (a GPUBlockReduction)."""
+    jpos, jpos0, thread = IndexVar("jpos"), IndexVar("jpos0"), IndexVar("thread")
+    return (stmt.pos(j, jpos, A[i, j])
+            .split(jpos, jpos0, thread, BLOCK_SIZE)
+            .reorder([i, thread, jpos0])
+            .parallelize(i, PU.GPUBlock, ORS.NoRaces)
+            .parallelize(thread, PU.GPUThread, ORS.Temporary))

@@ -43,6 +43,10 @@ def test_reference_spmv_schedules():
         ("spmv", (_lib.GEN_DEFAULT, 128, 0, 0, 0, 0))
     assert recognise(y, S.schedule_spmv_merge_gpu(st, A, pre, i, j)) == \
         ("spmv", (_lib.GEN_MERGE, 256, 8, 0, 0, 0))
+    assert recognise(y, S.schedule_spmv_block_rows_gpu(st, A, pre, i, j)) == \
+        ("spmv", (_lib.GEN_BLOCK_ROWS, 256, 0, 0, 0, 0))
+    assert recognise(y, S.schedule_spmv_block_rows_gpu(st, A, pre, i, j, 128)) == \
+        ("spmv", (_lib.GEN_BLOCK_ROWS, 128, 0, 0, 0, 0))
 
 
 def test_reference_spmm_schedule():

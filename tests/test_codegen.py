@@ -51,6 +51,21 @@ def test_ir_nnz_split_atomics():
     assert "#pragma unroll 8" in src
 
 
+def test_ir_block_rows_reduction():
+    ir = _lib.generate(codegen.standard("spmv_block_f64"), 1)
+    for unit in ("GPUBlock", "GPUThread", "GPUBlockReduction"):
+        assert unit in ir
+    assert "GPUWarp " not in ir
+    src = _lib.generate(codegen.standard("spmv_block_f64"))
+    assert "taco_hip_block_sum<256>(tthread_val)" in src
+    assert "if (thread == 0) { y_vals[i] = taco_red; }" in src
+    for bad_block in (96, 2048, 48):
+        with pytest.raises(_lib.TacoHipError, match="BLOCK_ROWS"):
+            _lib.generate(codegen.spec(_lib.OP_SPMV, _lib.GEN_BLOCK_ROWS, block_size=bad_block))
+    with pytest.raises(_lib.TacoHipError):
+        _lib.generate(codegen.spec(_lib.OP_SPMM, _lib.GEN_BLOCK_ROWS))
+
+
 def test_kernel_parameters_are_values_not_references():
     src = _lib.generate(codegen.standard("spmm_nnz_f64"))
     kernel_sig = [l for l in src.splitlines() if l.startswith("void computeDeviceKernel0(")][0]

@@ -30,7 +30,8 @@ def csr_tensor(name, dims, pos, crd, vals, insert=True):
     return A
 
 
-@pytest.mark.parametrize("schedule", ["default", "spmv_gpu", "rows", "thread_per_row", "merge"])
+@pytest.mark.parametrize("schedule", ["default", "spmv_gpu", "rows", "thread_per_row", "merge",
+                                      "block_rows"])
 def test_spmv_gpu_reference_inputs(schedule):
     d = np.load(os.path.join(GOLD, "ref_tests.npz"))
     i, j = taco.IndexVar("i"), taco.IndexVar("j")
@@ -51,6 +52,8 @@ def test_spmv_gpu_reference_inputs(schedule):
         stmt = S.schedule_spmv_thread_per_row_gpu(stmt, A, precomputed, i, j)
     elif schedule == "merge":
         stmt = S.schedule_spmv_merge_gpu(stmt, A, precomputed, i, j)
+    elif schedule == "block_rows":
+        stmt = S.schedule_spmv_block_rows_gpu(stmt, A, precomputed, i, j)
     y.compile(stmt)
     y.assemble()
     y.compute()

@@ -17,7 +17,8 @@ pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
-SPMV = ["spmv_default_f64", "spmv_rows_f64", "spmv_rows64_f64", "spmv_nnz_f64", "spmv_merge_f64"]
+SPMV = ["spmv_default_f64", "spmv_rows_f64", "spmv_rows64_f64", "spmv_nnz_f64", "spmv_merge_f64",
+        "spmv_block_f64"]
 SPMM = ["spmm_default_f64", "spmm_nnz_f64", "spmm_merge_f64"]
 
 
@@ -59,6 +60,21 @@ def test_generated_spmv(name, mname, A):
     assert rep["ok"], rep
     hand = G.spmv(A, x, G.sched(_lib.MERGE_PATH, 256, 8))
     assert oracle.error_report(y, hand, cond, 2e-12)["ok"]
+
+
+@pytest.mark.parametrize("block", [32, 64, 128, 1024])
+def test_generated_block_rows_sizes(block):
+    """The block reduction's wave-only (<= 64) and LDS (> 64) forms."""
+    name = f"spmv_block{block}"
+    if name not in KERNELS:
+        KERNELS[name] = codegen.Kernel(codegen.spec(_lib.OP_SPMV, _lib.GEN_BLOCK_ROWS,
+                                                    block_size=block))
+    for mname, A in MATS:
+        print(f"generated block rows {block} on {mname}", flush=True)
+        x = W.uniform(A.n, 5, W.STREAM_X)
+        y = run_spmv(name, A, x)
+        ye, cond = oracle.exact_spmv(A, x)
+        assert oracle.error_report(y, ye, cond, 1e-12)["ok"], (mname, block)
 
 
 @pytest.mark.parametrize("name", SPMV)
