@@ -527,3 +527,56 @@ class Tensor:
             self._release_output()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------- equals --
+
+def _stored(t):
+    """(coordinates as a (k, order) int64 array in storage order, values)."""
+    if t.format == CSR:
+        pos, crd, vals = t.to_csr()
+        rows = np.repeat(np.arange(t.dims[0], dtype=np.int64), np.diff(pos))
+        return np.stack([rows, crd.astype(np.int64)], axis=1), np.asarray(vals)
+    a = np.asarray(t.to_numpy()).reshape(-1)
+    coords = np.stack(np.unravel_index(np.arange(a.size), t.dims), axis=1).astype(np.int64)
+    return coords.reshape(a.size, len(t.dims)), a
+
+
+def _scalar_equals(a, b):
+    """taco's scalarEquals: |(a - b) / a| must not exceed 1e-5, computed in
+    double with a as the divisor (0/0 is NaN and passes; x/0 is inf and
+    fails)."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        diff = (np.asarray(a, np.float64) - np.asarray(b, np.float64)) / np.asarray(a, np.float64)
+    return ~(np.abs(diff) > 10e-6)
+
+
+def equals(a, b):
+    """taco's tensor predicate (src/tensor.cpp:1038-1149): same component
+    type, order and dimensions, then a merge over both tensors' stored
+    components in storage order.  Equal coordinates need scalarEquals;
+    a component stored in one tensor only must be zero.  Fill values are
+    zero for every tensor here."""
+    if a.dtype is not b.dtype or len(a.dims) != len(b.dims) or list(a.dims) != list(b.dims):
+        return False
+    ac, av = _stored(a)
+    bc, bv = _stored(b)
+    if ac.shape == bc.shape and np.array_equal(ac, bc):
+        return bool(np.all(_scalar_equals(av, bv)))
+    # general merge, in storage (lexicographic) order
+    ia = ib = 0
+    while ia < len(av) and ib < len(bv):
+        ka, kb = tuple(ac[ia]), tuple(bc[ib])
+        if ka != kb:
+            if av[ia] == 0:
+                ia += 1
+                continue
+            if bv[ib] == 0:
+                ib += 1
+                continue
+            return False
+        if not _scalar_equals(av[ia], bv[ib]):
+            return False
+        ia += 1
+        ib += 1
+    return bool(np.all(av[ia:] == 0) and np.all(bv[ib:] == 0))

@@ -108,3 +108,54 @@ def test_pack_and_state_errors():
         y.compute()
     with pytest.raises(taco.TacoError, match="format"):
         taco.Tensor("T", [2, 2, 2], taco.Format([taco.Dense, taco.Sparse, taco.Sparse]))
+
+
+def _dense(name, values, dtype=taco.Float64):
+    t = taco.Tensor(name, [len(values)], taco.Format([taco.Dense]), dtype)
+    t.set_values(np.array(values))
+    return t
+
+
+def _csr(name, shape, pos, crd, vals):
+    t = taco.Tensor(name, shape, taco.CSR)
+    t.set_csr(np.array(pos, np.int32), np.array(crd, np.int32), np.array(vals, np.float64))
+    return t
+
+
+def test_equals_scalar_rule():
+    """scalarEquals (tensor.cpp:1038-1045): |(a - b) / a| <= 1e-5, a divides."""
+    a = _dense("a", [1.0, -4.0, 0.0, 2.5])
+    assert taco.equals(a, _dense("b", [1.0, -4.0, 0.0, 2.5]))
+    assert taco.equals(a, _dense("b", [1.0 + 5e-6, -4.0 * (1 - 5e-6), 0.0, 2.5]))
+    assert not taco.equals(a, _dense("b", [1.0 + 2e-5, -4.0, 0.0, 2.5]))
+    # 0/0 is NaN and passes; x/0 is infinite and fails; b = 0 gives |1| > 1e-5
+    assert not taco.equals(a, _dense("b", [1.0, -4.0, 1e-300, 2.5]))
+    assert not taco.equals(_dense("a", [1e-300]), _dense("b", [0.0]))
+    # a divides, so the rule is not symmetric near the bound:
+    # 9.99995e-6 / 1 passes, 9.99995e-6 / (1 - 9.99995e-6) does not
+    assert taco.equals(_dense("a", [1.0]), _dense("b", [1.0 - 9.99995e-6]))
+    assert not taco.equals(_dense("a", [1.0 - 9.99995e-6]), _dense("b", [1.0]))
+
+
+def test_equals_shape_and_type():
+    a = _dense("a", [1.0, 2.0])
+    assert not taco.equals(a, _dense("b", [1.0, 2.0, 0.0]))
+    assert not taco.equals(a, _dense("b", [1.0, 2.0], taco.Float32))
+
+
+def test_equals_merge_skips_stored_zeros():
+    """equalsTyped (tensor.cpp:1056-1100): a component stored in one tensor
+    only must be zero; explicit zeros are skipped."""
+    a = _csr("a", [2, 3], [0, 2, 3], [0, 2, 1], [5.0, 0.0, 7.0])
+    b = _csr("b", [2, 3], [0, 1, 2], [0, 1], [5.0, 7.0])
+    assert taco.equals(a, b) and taco.equals(b, a)
+    c = _csr("c", [2, 3], [0, 2, 3], [0, 2, 1], [5.0, 1.0, 7.0])
+    assert not taco.equals(c, b) and not taco.equals(b, c)
+    # the same values at different coordinates
+    d = _csr("d", [2, 3], [0, 1, 2], [1, 1], [5.0, 7.0])
+    assert not taco.equals(d, b)
+    # trailing stored zeros are fine, trailing non-zeros are not
+    e = _csr("e", [2, 3], [0, 1, 3], [0, 1, 2], [5.0, 7.0, 0.0])
+    assert taco.equals(e, b)
+    f = _csr("f", [2, 3], [0, 1, 3], [0, 1, 2], [5.0, 7.0, 3.0])
+    assert not taco.equals(f, b)
