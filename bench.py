@@ -44,6 +44,24 @@ def host_cores():
     return max(1, min(n, 16))
 
 
+def host_cpu():
+    """CPU model and the cores this process may use (the baseline's host)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "usable_cores": usable, "logical_cpus": os.cpu_count()}
+
+
 def trimmed_mean(ts):
     ts = sorted(ts)
     cut = int(len(ts) * 0.1)
@@ -241,7 +259,7 @@ def bench_spmv(args, A, x, name, workload_desc):
     cpu = None
     if cpu_ts:
         cms = trimmed_mean(cpu_ts)
-        cpu = {"value": nbytes / (cms * 1e-3) / 1e9, "unit": "GB/s", "cores": cores,
+        cpu = {"value": nbytes / (cms * 1e-3) / 1e9, "unit": "GB/s", "cores": cores, "host": host_cpu(),
                "kind": kind, "ms": cms, "median_ms": float(np.median(cpu_ts)),
                "sample": f"whole workload, {len(cpu_ts)} timed compute() calls "
                          f"(10% trimmed mean), JIT excluded, inputs in host DRAM"}
@@ -326,7 +344,7 @@ def bench_spmm(args, A, dtype):
     cpu = None
     if cpu_ts:
         cms = trimmed_mean(cpu_ts)
-        cpu = {"value": flops / (cms * 1e-3) / 1e9, "unit": "GFLOP/s", "cores": cores,
+        cpu = {"value": flops / (cms * 1e-3) / 1e9, "unit": "GFLOP/s", "cores": cores, "host": host_cpu(),
                "kind": kind, "ms": cms, "sample": f"whole workload, {len(cpu_ts)} calls"}
     return {
         "metric": "SpMM GFLOP/s", "value": gflops, "unit": "GFLOP/s", "n_gpus": 1,
@@ -403,7 +421,7 @@ def bench_spgemm(args):
     cpu = None
     if cpu_ts:
         cms = trimmed_mean(cpu_ts)
-        cpu = {"value": nnzc / (cms * 1e-3), "unit": "nnz(C)/s", "cores": cores,
+        cpu = {"value": nnzc / (cms * 1e-3), "unit": "nnz(C)/s", "cores": cores, "host": host_cpu(),
                "kind": kind, "ms": cms, "sample": "whole workload, assemble + compute"}
     return {
         "metric": "SpGEMM nnz(C)/s (assemble + compute)", "unit": "nnz(C)/s",
