@@ -132,3 +132,17 @@ def test_autotune_batch(wl):
     b.run()
     torch.cuda.synchronize()
     check_spmv(wl, range(wl.ninst))
+
+
+@pytest.mark.parametrize("ninst", [1, 3, 11])
+def test_batch_fewer_instances_than_queues(ninst):
+    """Per-XCD queues with empty queues: every task still runs exactly once."""
+    w = BT.BatchedWorkload(ninst=ninst, rmin=70, rmax=300, seed=20 + ninst, width=32,
+                           keep_host=True)
+    for lanes in (0, 4):
+        w.y.fill_(float("nan"))
+        b = BT.Batch(w, 0, np.full(w.ninst, lanes, np.int32), rows_per_task=16)
+        b.run()
+        torch.cuda.synchronize()
+        check_spmv(w, range(w.ninst))
+        del b

@@ -220,6 +220,27 @@ def test_spgemm(name, A, B):
         assert np.array_equal(vals, ovals)
 
 
+@pytest.mark.parametrize("seed", range(12))
+def test_spgemm_random_shapes(seed):
+    """Seeded sweep of row-length mixes around the 64-candidate boundary: A
+    rows of 0..100 entries, B rows that are often empty or longer than 64."""
+    rng = np.random.default_rng(100 + seed)
+    m, k, n = (int(v) for v in rng.integers(1, 1500, 3))
+    ca = rng.integers(0, 12, m).astype(np.int32)
+    ca[rng.random(m) < 0.05] = rng.integers(60, 101)
+    cb = rng.integers(0, 10, k).astype(np.int32)
+    cb[rng.random(k) < rng.random()] = 0
+    cb[rng.random(k) < 0.02] = 70
+    A = W.csr_from_counts(m, k, np.minimum(ca, k), 200 + seed)
+    B = W.csr_from_counts(k, n, np.minimum(cb, n), 300 + seed)
+    print(f"spgemm sweep seed={seed} m={m} k={k} n={n} nnzA={A.nnz} nnzB={B.nnz}", flush=True)
+    pos, crd, vals = G.spgemm(A, B)
+    opos, ocrd = oracle.spgemm_assemble(A, B)
+    assert np.array_equal(pos, opos) and np.array_equal(crd, ocrd)
+    ex, cond = oracle.exact_spgemm_values(A, B, opos, ocrd)
+    assert oracle.error_report(vals, ex, cond, TOL["f64"])["ok"]
+
+
 def test_spgemm_known_answer():
     p, c, v = KA.coo_to_csr(5, 5, KA.SPMM_B)
     A = W.Csr(5, 5, p, c, v)
