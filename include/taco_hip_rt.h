@@ -421,7 +421,7 @@ __device__ inline int32_t taco_hip_spgemm_row_symbolic(
     const uint64_t heads = __ballot(head);
     if (FILL && head) {
       const int rank = __popcll(heads & ((1ull << lane) - 1));
-      ccrd[cpos_i + rank] = s;
+      __builtin_nontemporal_store(s, ccrd + cpos_i + rank); /* written once */
     }
     return __popcll(heads);
   }
@@ -433,7 +433,7 @@ __device__ inline int32_t taco_hip_spgemm_row_symbolic(
   }
   if (FILL) {
     taco_hip_row_hash_sort(w);
-    for (int q = lane; q < cnt; q += 64) ccrd[cpos_i + q] = w->keys[q];
+    for (int q = lane; q < cnt; q += 64) __builtin_nontemporal_store(w->keys[q], ccrd + cpos_i + q);
   }
   return cnt;
 }
@@ -485,7 +485,7 @@ __device__ inline void taco_hip_spgemm_row_numeric(
     }
     if (head) {
       const int rank = __popcll(heads & ((1ull << lane) - 1));
-      if (rank < nc) cvals[c0 + rank] = s;
+      if (rank < nc) __builtin_nontemporal_store(s, cvals + c0 + rank); /* written once */
     }
     return;
   }
@@ -521,7 +521,7 @@ __device__ inline void taco_hip_spgemm_row_numeric(
       }
     }
   }
-  for (int q = lane; q < nc; q += 64) cvals[c0 + q] = acc[q];
+  for (int q = lane; q < nc; q += 64) __builtin_nontemporal_store(acc[q], cvals + c0 + q);
 }
 
 #endif

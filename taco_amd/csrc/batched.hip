@@ -194,7 +194,8 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_spmm(
           const float a = stream_load(iv + q);
           acc += a * *(const f32x4*)(iB + (int64_t)stream_load(ic + q) * K + 4 * lane);
         }
-        *(f32x4*)(iC + (int64_t)r * K + 4 * lane) = acc;
+        // non-temporal: C is written once, B rows keep the caches
+        __builtin_nontemporal_store(acc, (f32x4*)(iC + (int64_t)r * K + 4 * lane));
       }
       return;
     }

@@ -275,7 +275,8 @@ __global__ __launch_bounds__(256) void k_spmm_merge(
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
         const int k = lane + c * G;
-        if (k < K) C[r * K + k] = acc[c];
+        // non-temporal: C is written once, B rows keep the caches
+        if (k < K) __builtin_nontemporal_store(acc[c], C + r * K + k);
         acc[c] = 0;
       }
       partial = false;

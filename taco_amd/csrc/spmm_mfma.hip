@@ -121,7 +121,10 @@ __global__ __launch_bounds__(256) void k_spmm_mfma_bf16(
       if (lr >= rows) continue;
       float* dst = C + (r0 + lr) * K + col;
       if (nparts == 1) {
-        *(float4*)dst = make_float4(acc[0][r], acc[1][r], acc[2][r], acc[3][r]);
+        // Non-temporal: C is written once; keeping it out of the caches
+        // leaves them to the gathered B rows.
+        const v4f32 o = {acc[0][r], acc[1][r], acc[2][r], acc[3][r]};
+        __builtin_nontemporal_store(o, (v4f32*)dst);
       } else {
         atomic_add(dst + 0, acc[0][r]);
         atomic_add(dst + 1, acc[1][r]);
