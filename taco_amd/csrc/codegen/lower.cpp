@@ -229,6 +229,28 @@ Stmt spmv_nnz_split(const taco_hip_gen_spec& s, DType vt, const Csr& A,
                             if_then(gte(fA, load(A.pos, add(ip, i32(1)))), advance),
                             assign(t, add(t, mul(load(A.vals, fA), load(x.vals, f))))}))}),
       LoopKind::Serial, ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces, K);
+  // A chunk wholly inside [0, nnz) drops the per-entry guard and reads its K
+  // column indices, values and x entries into registers before the row
+  // walk: straight-line loads that the compiler merges into vector loads,
+  // with every gather in flight at once.  Same products, same order.
+  Expr cc = var("chunk_crd", DType::Int32, true), cv = var("chunk_vals", vt, true);
+  Expr cx = var("chunk_x", vt, true);
+  Stmt prefetch = for_loop(
+      tnz, i32(0), i32(K), i32(1),
+      block({store(cc, tnz, load(A.crd, add(f0, tnz))),
+             store(cv, tnz, load(A.vals, add(f0, tnz)))}),
+      LoopKind::Serial, ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces, K);
+  Stmt gather = for_loop(tnz, i32(0), i32(K), i32(1),
+                         store(cx, tnz, load(x.vals, load(cc, tnz))), LoopKind::Serial,
+                         ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces, K);
+  Stmt walk = for_loop(
+      tnz, i32(0), i32(K), i32(1),
+      block({decl(fA, add(f0, tnz)),
+             if_then(gte(fA, load(A.pos, add(ip, i32(1)))), advance),
+             assign(t, add(t, mul(load(cv, tnz), load(cx, tnz))))}),
+      LoopKind::Serial, ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces, K);
+  Stmt full_chunk = block({allocate(cc, i32(K)), allocate(cv, i32(K)), allocate(cx, i32(K)),
+                           prefetch, gather, walk});
   Stmt thread_body = block({
       decl(f0, add(add(mul(block_v, i32(per_tb)), mul(warp, i32(per_warp))),
                    mul(thread, i32(K)))),
@@ -237,7 +259,8 @@ Stmt spmv_nnz_split(const taco_hip_gen_spec& s, DType vt, const Csr& A,
                                    {A.pos, load(starts, block_v),
                                     load(starts, add(block_v, i32(1))), f0},
                                    DType::Int32)),
-                     decl(t, zero(vt)), chunk,
+                     decl(t, zero(vt)),
+                     if_then(lte(f0, sub(nnz, i32(K))), full_chunk, chunk),
                      store_add(y.vals, ip, t, true, ParallelUnit::GPUWarp)}))});
   return for_loop(
       block_v, i32(0), nb, i32(1),
