@@ -429,6 +429,34 @@ Stmt spmm_nnz_split(const taco_hip_gen_spec& s, DType vt, const Csr& A,
              if_then(lt(k, Bm.dim2),
                      block({decl(ip, ip0), decl(t, zero(vt)), entries,
                             store_add(C.vals, cidx(ip), t, true, ParallelUnit::GPUThread)}))}));
+  // A chunk wholly inside [0, nnz): its P column indices and values are read
+  // once into registers (shared by the lane's D columns), then for each
+  // column all P B-row gathers are issued before the row walk.  Same
+  // products, same order as the guarded loop.
+  Expr cc = var("chunk_crd", DType::Int32, true), cv = var("chunk_vals", vt, true);
+  Expr cb = var("chunk_b", vt, true);
+  Stmt prefetch = for_loop(
+      nz, i32(0), i32(P), i32(1),
+      block({store(cc, nz, load(A.crd, add(f0, nz))), store(cv, nz, load(A.vals, add(f0, nz)))}),
+      LoopKind::Serial, ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces, P);
+  Stmt gather = for_loop(
+      nz, i32(0), i32(P), i32(1),
+      store(cb, nz, load(Bm.vals, add(mul(i64(load(cc, nz)), i64(Bm.dim2)), i64(k)))),
+      LoopKind::Serial, ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces, P);
+  Stmt walk = for_loop(
+      nz, i32(0), i32(P), i32(1),
+      block({decl(fA, add(f0, nz)),
+             if_then(gte(fA, load(A.pos, add(ip, i32(1)))), advance),
+             assign(t, add(t, mul(load(cv, nz), load(cb, nz))))}),
+      LoopKind::Serial, ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces, P);
+  Stmt full_cols = for_loop(
+      dv, i32(0), i32(D), i32(1),
+      block({decl(k, add(mul(dv, i32(W)), thread)),
+             if_then(lt(k, Bm.dim2),
+                     block({decl(ip, ip0), decl(t, zero(vt)), allocate(cb, i32(P)), gather,
+                            walk,
+                            store_add(C.vals, cidx(ip), t, true, ParallelUnit::GPUThread)}))}));
+  Stmt full = block({allocate(cc, i32(P)), allocate(cv, i32(P)), prefetch, full_cols});
   Stmt thread_body = block({
       decl(f0, add(mul(block_v, i32(per_tb)), mul(warp, i32(P)))),
       if_then(lt(f0, nnz),
@@ -436,7 +464,7 @@ Stmt spmm_nnz_split(const taco_hip_gen_spec& s, DType vt, const Csr& A,
                                     {A.pos, load(starts, block_v),
                                      load(starts, add(block_v, i32(1))), f0},
                                     DType::Int32)),
-                     cols}))});
+                     if_then(lte(f0, sub(nnz, i32(P))), full, cols)}))});
   return for_loop(
       block_v, i32(0), nb, i32(1),
       for_loop(warp, i32(0), i32(warps), i32(1),
