@@ -236,6 +236,21 @@ __device__ inline K taco_hip_wave_sort(K v) {
   return v;
 }
 
+/* Workgroups for a grid-strided row loop over m rows, 4 rows (waves) per
+ * workgroup: at most 4 workgroups per CU, which every row kernel here keeps
+ * resident at once. */
+static inline int32_t taco_hip_row_grid(int32_t m) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+    cus = n > 0 ? n : 256;
+  }
+  const int64_t want = ((int64_t)m + 3) / 4;
+  return (int32_t)(want < 4 * (int64_t)cus ? want : 4 * (int64_t)cus);
+}
+
 /* ---- SpGEMM rows, one 64-lane wave per row of C = A * B ----
  * Pattern of row i: sorted distinct structural columns of the B rows named by
  * A's row i.  Values: products summed per column in encounter order (jA

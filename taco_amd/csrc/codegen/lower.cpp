@@ -590,6 +590,16 @@ std::vector<Stmt> lower_spmm(const taco_hip_gen_spec& s, DType vt) {
 // One 64-lane wave per row of C, four waves per workgroup: the GPU form of
 // taco's CPU assemble/compute (workspace scatter + sorted index list
 // becomes a wave sort or LDS hash set; the pos prefix sum a device scan).
+// Workgroups for the compute row loop (taco_hip_row_grid: one per 4 rows,
+// at most 4 per CU).  A row takes a few thousand cycles, too short for one
+// workgroup per four rows to keep the CUs occupied, so rows are strided
+// over a grid the device holds at once.
+Stmt row_grid_decl(const Csr& A, Expr nb) {
+  return decl(nb, call("taco_hip_row_grid", {A.dim1}, DType::Int32));
+}
+
+// Assemble: one workgroup per four rows.  Compute: rows strided over a grid
+// the device holds at once (nb workgroups, taco_hip_row_grid).
 Stmt row_waves(const Csr& A, Expr i, Stmt body) {
   Expr r0 = var("r0", DType::Int32), w = var("w", DType::Int32);
   Expr lane = var("lane", DType::Int32);
@@ -599,6 +609,19 @@ Stmt row_waves(const Csr& A, Expr i, Stmt body) {
                for_loop(lane, i32(0), i32(64), i32(1),
                         block({decl(i, add(mul(r0, i32(4)), w)),
                                if_then(lt(i, A.dim1), body)}),
+                        LoopKind::Serial, ParallelUnit::GPUThread, OutputRaceStrategy::NoRaces),
+               LoopKind::Serial, ParallelUnit::GPUWarp, OutputRaceStrategy::NoRaces),
+      LoopKind::Serial, ParallelUnit::GPUBlock, OutputRaceStrategy::NoRaces);
+}
+
+Stmt strided_row_waves(const Csr& A, Expr i, Expr nb, Stmt body) {
+  Expr r0 = var("r0", DType::Int32), w = var("w", DType::Int32);
+  Expr lane = var("lane", DType::Int32);
+  return for_loop(
+      r0, i32(0), nb, i32(1),
+      for_loop(w, i32(0), i32(4), i32(1),
+               for_loop(lane, i32(0), i32(64), i32(1),
+                        for_loop(i, add(mul(r0, i32(4)), w), A.dim1, mul(nb, i32(4)), body),
                         LoopKind::Serial, ParallelUnit::GPUThread, OutputRaceStrategy::NoRaces),
                LoopKind::Serial, ParallelUnit::GPUWarp, OutputRaceStrategy::NoRaces),
       LoopKind::Serial, ParallelUnit::GPUBlock, OutputRaceStrategy::NoRaces);
@@ -660,7 +683,9 @@ std::vector<Stmt> lower_spgemm(const taco_hip_gen_spec& s) {
   c.push_back(decl(Cpos, cprop(TensorProperty::Indices, 1, 0, DType::Int32, true)));
   c.push_back(decl(Ccrd, cprop(TensorProperty::Indices, 1, 1, DType::Int32, true)));
   c.push_back(decl(Cvals, cprop(TensorProperty::Values, 0, 0, vt, true)));
-  c.push_back(row_waves(A, i,
+  Expr nb = var("row_blocks", DType::Int32);
+  c.push_back(row_grid_decl(A, nb));
+  c.push_back(strided_row_waves(A, i, nb,
                         evaluate(call("taco_hip_spgemm_row_numeric",
                                       {A.pos, A.crd, A.vals, B.pos, B.crd, B.vals, i,
                                        Cpos, Ccrd, Cvals}, DType::Int32))));
