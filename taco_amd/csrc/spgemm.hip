@@ -51,7 +51,8 @@ __global__ __launch_bounds__(256) void k_spgemm_symbolic(
 }
 
 // Numeric pass on the assembled pattern.  Rows are strided over a grid the
-// device holds at once (row_grid): one workgroup per four rows left about 4
+// device holds at once (taco_hip_row_grid, 4 workgroups per CU; the kernel
+// fits 5): one workgroup per four rows left about 4
 // waves resident per CU, since a row takes only a few thousand cycles
 // (0.56 -> 0.51 ms at 500k rows).  The symbolic passes measured faster with
 // one workgroup per four rows, where running waves stream through
@@ -69,20 +70,6 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(
                                 cpos, ccrd, cvals);
 }
 
-// Workgroups for m rows: one per 4 rows, at most as many as the device
-// holds at once for `kernel` (a larger grid would run its surplus
-// workgroups, each with its full share of rows, after the first ones).
-unsigned row_grid(const void* kernel, int64_t m) {
-  int dev = 0, cus = 256, per_cu = 4;
-  if (hipGetDevice(&dev) == hipSuccess)
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  const int64_t want = (m + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int64_t cap = (int64_t)(cus > 0 ? cus : 256) * per_cu;
-  return (unsigned)(want < cap ? want : cap);
-}
 
 // ---- counts -> pos (exclusive prefix sum with pos[m] = total) ----
 constexpr int kScanBlock = 256;
@@ -281,7 +268,7 @@ extern "C" int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,
   if (int rc = view_csr(C, "C", &c)) return rc;
   if (a.m == 0 || c.nnz == 0) return finish("taco_hip_spgemm_compute");
   hipStream_t st = current_stream();
-  const unsigned grid = row_grid((const void*)k_spgemm_numeric, a.m);
+  const unsigned grid = (unsigned)taco_hip_row_grid(a.m);
   k_spgemm_numeric<<<grid, 256, 0, st>>>(
       a.m, a.pos, a.crd, (const double*)a.vals, b.pos, b.crd,
       (const double*)b.vals, c.pos, c.crd, (double*)c.vals);
