@@ -6,7 +6,7 @@ schedules of ``autotune.SPMV_CANDIDATES`` and the 11 SpMM schedules of
 ``autotune.spmm_candidates`` — plus a predicted log time.  The network is
 trained on the GPU by the library's own forward, backward and Adam kernels
 (``taco_hip_mlp_*``) on timings the exhaustive autotuner measures; the
-autotuner stays its fallback and judge.  The selector never changes a
+autotuner stays its fallback and arbiter.  The selector never changes a
 result: it only picks which of the implemented schedules runs, and every
 schedule computes the same values to the value tolerance.
 

@@ -135,7 +135,7 @@ def main():
     t.add_argument("--reps", type=int, default=3)
     t.add_argument("--progress", type=int, default=50)
     t.add_argument("--op", type=int, default=-1, help="only problems of this op (0/1)")
-    t.add_argument("--out", default="gpurun_out")
+    t.add_argument("--out", default=".")
     r = sub.add_parser("train")
     r.add_argument("--db", nargs="+", required=True)
     r.add_argument("--out", required=True)
