@@ -133,7 +133,7 @@ class Mlp:
         return float(loss.value) if want_loss else None
 
     def __del__(self):
-        if getattr(self, "_p", None) and _lib._lib is not None:
+        if getattr(self, "_p", None) and _lib is not None and _lib._lib is not None:
             _lib._lib.taco_hip_mlp_destroy(self._p)
             self._p = None
 
