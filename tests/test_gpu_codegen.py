@@ -120,6 +120,39 @@ def test_generated_spmm(name, k):
     assert rep["ok"], rep
 
 
+@pytest.mark.parametrize("name", ["spmv_default_f32", "spmv_nnz_f32"])
+def test_generated_spmv_f32(name):
+    """f32 modules, including the nnz-split full-chunk path, against the FP64
+    oracle on the f32-rounded operands (tolerance 1e-5)."""
+    for mname, A in MATS:
+        print(f"generated {name} on {mname}", flush=True)
+        A32 = W.Csr(A.m, A.n, A.pos, A.crd, A.vals.astype(np.float32).astype(np.float64))
+        x = W.uniform(A.n, 6, W.STREAM_X).astype(np.float32).astype(np.float64)
+        a = G.csr_desc(A32, "f32")
+        xd = G.dev(x, torch.float32)
+        y = torch.full((A.m,), float("nan"), dtype=torch.float32, device="cuda")
+        kernel(name).compute(D.dense(y, "f32"), a, D.dense(xd, "f32"))
+        torch.cuda.synchronize()
+        ye, cond = oracle.exact_spmv(A32, x)
+        rep = oracle.error_report(y.cpu().numpy().astype(np.float64), ye, cond, 1e-5)
+        assert rep["ok"], (mname, rep)
+
+
+@pytest.mark.parametrize("k", [32, 128])
+def test_generated_spmm_f32(k):
+    A = W.csr_powerlaw(5000, 4000, seed=35, kmax=3000)
+    A32 = W.Csr(A.m, A.n, A.pos, A.crd, A.vals.astype(np.float32).astype(np.float64))
+    B = W.uniform(A.n * k, 35, W.STREAM_B).reshape(A.n, k).astype(np.float32).astype(np.float64)
+    a = G.csr_desc(A32, "f32")
+    Bd = G.dev(B, torch.float32)
+    C = torch.full((A.m, k), float("nan"), dtype=torch.float32, device="cuda")
+    kernel("spmm_nnz_f32").compute(D.dense(C, "f32"), a, D.dense(Bd, "f32"))
+    torch.cuda.synchronize()
+    Ce, cond = oracle.exact_spmm(A32, B)
+    rep = oracle.error_report(C.cpu().numpy().astype(np.float64), Ce, cond, 1e-5)
+    assert rep["ok"], rep
+
+
 def test_generated_spmm_reference_inputs_bit_exact():
     d = np.load(os.path.join(GOLD, "ref_tests.npz"))
     A = W.Csr(102, 103, d["spmmgpu_pos"], d["spmmgpu_crd"], d["spmmgpu_vals"])
