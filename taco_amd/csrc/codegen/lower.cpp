@@ -515,13 +515,38 @@ Stmt spmm_merge(const taco_hip_gen_spec& s, DType vt, const Csr& A, const Dense&
                     LoopKind::Serial, ParallelUnit::NotParallel,
                     OutputRaceStrategy::IgnoreRaces, D);
   };
-  Stmt gather = for_loop(
-      u, i32(0), cnt, i32(1),
+  // Eight broadcast entries at a time: their B rows are gathered together,
+  // then added in entry order (the order of the one-at-a-time tail loop).
+  constexpr int kBatch = 8;
+  Expr u0 = var("u0", DType::Int32), q = var("q", DType::Int32);
+  Expr jj = var("batch_j", DType::Int32, true), aa = var("batch_a", vt, true);
+  Expr bb = var("batch_b", vt, true);
+  Stmt batch_bcast = for_loop(
+      q, i32(0), i32(kBatch), i32(1),
+      block({store(jj, q, call("taco_hip_team_bcast", {jl, add(u0, q), i32(W)}, DType::Int32)),
+             store(aa, q, call("taco_hip_team_bcast", {al, add(u0, q), i32(W)}, vt))}),
+      LoopKind::Serial, ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces, kBatch);
+  Stmt batch_cols = each_col(block({
+      allocate(bb, i32(kBatch)),
+      for_loop(q, i32(0), i32(kBatch), i32(1),
+               store(bb, q, load(Bm.vals, add(mul(i64(load(jj, q)), i64(Bm.dim2)), i64(k)))),
+               LoopKind::Serial, ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces,
+               kBatch),
+      for_loop(q, i32(0), i32(kBatch), i32(1),
+               store(acc, dv, add(load(acc, dv), mul(load(aa, q), load(bb, q)))),
+               LoopKind::Serial, ParallelUnit::NotParallel, OutputRaceStrategy::IgnoreRaces,
+               kBatch)}));
+  Stmt batches = for_loop(u0, i32(0), sub(cnt, i32(kBatch - 1)), i32(kBatch),
+                          block({allocate(jj, i32(kBatch)), allocate(aa, i32(kBatch)),
+                                 batch_bcast, batch_cols}));
+  Stmt tail = for_loop(
+      u, mul(div(cnt, i32(kBatch)), i32(kBatch)), cnt, i32(1),
       block({decl(j, call("taco_hip_team_bcast", {jl, u, i32(W)}, DType::Int32)),
              decl(a, call("taco_hip_team_bcast", {al, u, i32(W)}, vt)),
              each_col(store(acc, dv, add(load(acc, dv),
                                          mul(a, load(Bm.vals, add(mul(i64(j), i64(Bm.dim2)),
                                                                   i64(k)))))))}));
+  Stmt gather = block({batches, tail});
   Stmt chunks = for_loop(
       zc, fA, zstop, i32(W),
       block({decl(cnt, min(i32(W), sub(zstop, zc))),
