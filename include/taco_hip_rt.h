@@ -99,9 +99,20 @@ __device__ __host__ inline int32_t taco_hip_mergePathSearch(const int32_t* pos,
 }
 
 /* Value of lane `idx` of the caller's team of `width` lanes (idx uniform
- * across the team). */
+ * across the team).  A whole-wave team reads it with v_readlane into a
+ * scalar; narrower teams use an LDS permute (__shfl). */
 template <typename T>
 __device__ inline T taco_hip_team_bcast(T v, int32_t idx, int32_t width) {
+  if (width == 64) {
+    if constexpr (sizeof(T) == 8) {
+      const uint64_t b = __builtin_bit_cast(uint64_t, v);
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, idx);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), idx);
+      return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    } else if constexpr (sizeof(T) == 4) {
+      return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), idx));
+    }
+  }
   return __shfl(v, idx, width);
 }
 
