@@ -66,6 +66,33 @@ def test_ir_block_rows_reduction():
         _lib.generate(codegen.spec(_lib.OP_SPMM, _lib.GEN_BLOCK_ROWS))
 
 
+def test_nnz_split_full_chunk_paths():
+    """Full chunks skip the per-entry guard and stage their entries (and, for
+    SpMM, their B values) in registers; partial chunks keep the guard."""
+    src = _lib.generate(codegen.standard("spmv_nnz_f64"))
+    assert "if ((fpos0 <= (A_nnz - 8)))" in src
+    assert "chunk_crd[thread_nz] = A2_crd[(fpos0 + thread_nz)];" in src
+    assert "chunk_x[thread_nz] = x_vals[chunk_crd[thread_nz]];" in src
+    assert "if ((fposA < A_nnz))" in src  # the guarded partial chunk
+    src = _lib.generate(codegen.standard("spmm_nnz_f64"))
+    assert "double chunk_b[8];" in src and "if ((fpos0 <= (A_nnz - 8)))" in src
+
+
+def test_merge_spmm_batches_gathers():
+    src = _lib.generate(codegen.standard("spmm_merge_f64"))
+    assert "for (int32_t u0 = 0; u0 < (cnt - 7); u0 += 8)" in src
+    assert "batch_b[q] = B_vals[" in src
+    assert "for (int32_t u = ((cnt / 8) * 8); u < cnt; u += 1)" in src  # tail
+
+
+def test_spgemm_compute_strides_rows():
+    src = _lib.generate(codegen.standard("spgemm_f64"))
+    assert "int32_t row_blocks = taco_hip_row_grid(A1_dimension);" in src
+    assert "i += (row_blocks * 4)" in src
+    # assemble keeps one workgroup per four rows
+    assert src.count("taco_hip_row_grid") == 1
+
+
 def test_kernel_parameters_are_values_not_references():
     src = _lib.generate(codegen.standard("spmm_nnz_f64"))
     kernel_sig = [l for l in src.splitlines() if l.startswith("void computeDeviceKernel0(")][0]
