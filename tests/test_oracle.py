@@ -206,6 +206,68 @@ def test_exact_oracle_against_fractions():
         assert abs(cond[i] - c) <= 1e-15 * max(c, 1e-300)
 
 
+def test_exact_spmm_against_fractions():
+    """exact_spmm against exact rational arithmetic, every (i, k), on a
+    seeded power-law matrix with empty, short and long rows."""
+    from fractions import Fraction
+    P = W.csr_powerlaw(50, 70, seed=21, kmax=70)
+    # one empty row appended after short and long ones
+    A = W.Csr(51, 70, np.append(P.pos, P.pos[-1]).astype(np.int32), P.crd, P.vals)
+    assert np.diff(A.pos).max() > 20 and np.diff(A.pos)[-1] == 0
+    K = 5
+    B = W.uniform(70 * K, 21, W.STREAM_B).reshape(70, K)
+    C, cond = oracle.exact_spmm(A, B)
+    for i in range(A.m):
+        for k in range(K):
+            terms = [Fraction(A.vals[p]) * Fraction(B[A.crd[p], k])
+                     for p in range(A.pos[i], A.pos[i + 1])]
+            assert C[i, k] == float(sum(terms, Fraction(0))), (i, k)
+            c = sum(abs(float(t)) for t in terms)
+            assert abs(cond[i, k] - c) <= 1e-15 * max(c, 1e-300), (i, k)
+
+
+def test_exact_spmm_columns_are_exact_spmv():
+    """Column k of exact_spmm(A, B) is exact_spmv(A, B[:, k]), bit for bit,
+    values and condition sums, on a 4,000-row power-law matrix."""
+    A = W.csr_powerlaw(4000, 3500, seed=22, kmax=3000)
+    K = 9
+    B = W.uniform(A.n * K, 22, W.STREAM_B).reshape(A.n, K)
+    C, cond = oracle.exact_spmm(A, B)
+    for k in range(K):
+        y, c = oracle.exact_spmv(A, np.ascontiguousarray(B[:, k]))
+        assert np.array_equal(C[:, k], y), k
+        assert np.array_equal(cond[:, k], c), k
+
+
+def test_exact_spmm_integer_golden():
+    """The reference's SpMM GPU test inputs (tests-scheduling-eval.cpp:
+    1258-1307) hold small integers, so every correct summation order gives
+    the golden written by the real taco exactly."""
+    d = np.load(os.path.join(GOLD, "ref_tests.npz"))
+    A = W.Csr(102, 103, d["spmmgpu_pos"], d["spmmgpu_crd"], d["spmmgpu_vals"])
+    C, _ = oracle.exact_spmm(A, d["spmmgpu_B"])
+    assert np.array_equal(C, d["spmmgpu_C"])
+
+
+def test_exact_spmm_gate_rejects_transposed_b():
+    """A result computed with B indexed as if column-major (B[k*n + j]
+    instead of B[j*K + k]) fails the 1e-12 gate against exact_spmm."""
+    A = W.csr_powerlaw(3000, 3000, seed=23, kmax=800)
+    K = 16
+    B = W.uniform(A.n * K, 23, W.STREAM_B).reshape(A.n, K)
+    Ce, cond = oracle.exact_spmm(A, B)
+    assert oracle.error_report(oracle.spmm(A, B), Ce, cond, 1e-12)["ok"]
+    # B'[j, k] = flat(B)[k * n + j]: the transposed-index mistake.
+    Bt = np.ascontiguousarray(B.ravel().reshape(K, A.n).T)
+    assert not oracle.error_report(oracle.spmm(A, Bt), Ce, cond, 1e-12)["ok"]
+    # and a dropped last column term, on one long row
+    r = int(np.argmax(np.diff(A.pos)))
+    Cd = oracle.spmm(A, B)
+    p = A.pos[r + 1] - 1
+    Cd[r, K - 1] -= A.vals[p] * B[A.crd[p], K - 1]
+    assert not oracle.error_report(Cd, Ce, cond, 1e-12)["ok"]
+
+
 def test_exact_spgemm_values_against_fractions():
     from fractions import Fraction
     A = W.csr_fixed(30, 40, 5, 11)
