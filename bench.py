@@ -23,6 +23,18 @@ import os
 import sys
 import time
 
+# The CPU baseline's OpenMP threads are pinned one per core, packed; the
+# OpenMP runtimes read these when they start, so they are set before any
+# library that carries one (torch, the CPU oracle) is loaded.  Once an
+# OpenMP runtime starts with binding on, it pins the calling thread to one
+# core, so the process's CPU set is read here, before that.
+try:
+    AFFINITY_CPUS = len(os.sched_getaffinity(0))
+except AttributeError:
+    AFFINITY_CPUS = os.cpu_count() or 1
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -30,18 +42,51 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+SPMM_K = 64
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def host_cores():
+def cgroup_cpu_quota():
+    """CPUs granted by the cgroup's CPU quota (cgroup v2 cpu.max, v1
+    cfs_quota/period), or None when unlimited or unreadable."""
     try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            return max(1, int(int(q) // int(per)))
+        return None
+    except (OSError, ValueError):
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us") as f:
+            q = int(f.read())
+        with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as f:
+            per = int(f.read())
+        return max(1, q // per) if q > 0 else None
+    except (OSError, ValueError):
+        return None
+
+
+def affinity_cpus():
+    return AFFINITY_CPUS
+
+
+def host_cores():
+    """Every host core this process may run on: the CPUs of its affinity
+    mask, limited by the cgroup CPU quota (more threads than the quota
+    would only be throttled)."""
+    n = affinity_cpus()
+    q = cgroup_cpu_quota()
+    return max(1, min(n, q) if q else n)
+
+
+def cpu_threading():
+    return {"affinity_cpus": affinity_cpus(), "cgroup_quota_cpus": cgroup_cpu_quota(),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "OMP_PLACES": os.environ.get("OMP_PLACES")}
 
 
 def host_cpu():
@@ -55,11 +100,9 @@ def host_cpu():
                     break
     except OSError:
         pass
-    try:
-        usable = len(os.sched_getaffinity(0))
-    except AttributeError:
-        usable = os.cpu_count() or 1
-    return {"model": model, "usable_cores": usable, "logical_cpus": os.cpu_count()}
+    out = {"model": model, "logical_cpus": os.cpu_count()}
+    out.update(cpu_threading())
+    return out
 
 
 def trimmed_mean(ts):
@@ -186,12 +229,17 @@ def sched(_lib, algo, block=0, items=0, group=0, rows=0):
     return s
 
 
-def bench_spmv(args, A, x, name, workload_desc):
+def bench_spmv(args, A, x, name, workload_desc, spmm=None):
+    """SpMV on one matrix.  spmm = spmm_inputs(...) adds the SpMM legs (f64,
+    f32, bf16 with K = 64 on the same matrix) to the same JSON line."""
+    import oracle
     from taco_amd import workloads as W
     y_ref, cpu_ts, cores, kind = (None, [], 0, None)
     if not args.no_cpu:
         log(f"cpu baseline: taco SpMV on {host_cores()} cores ...")
         y_ref, cpu_ts, cores, kind = cpu_baseline_spmv(A, x, args.cpu_reps)
+    oracle.set_threads(host_cores())
+    exact_y = oracle.exact_spmv(A, x)
     torch, L, _lib = gpu_setup()
     from taco_amd import descriptors as D
     pos = torch.from_numpy(A.pos).cuda()
@@ -213,22 +261,27 @@ def bench_spmv(args, A, x, name, workload_desc):
     plan = _lib.SpmvPlan(a, args.parts, 0)
     torch.cuda.synchronize()
     plan_ms = (time.perf_counter() - t0) * 1e3
-    log(f"gpu: plan {plan_ms:.1f} ms {plan.info()}; {args.warmup} warmup + {args.steps} steps ...")
+    plan_info = plan.info()
+    log(f"gpu: plan {plan_ms:.1f} ms {plan_info}; {args.warmup} warmup + {args.steps} steps ...")
     # Events around the main kernel of every timed step (same stream).
     ms, wall_ms = timer.run(lambda: plan.compute(ydsc, xdsc, args.unroll), args.steps, args.warmup,
                             before_timed=lambda: plan.time_kernel(args.steps))
     kernel_ms, kernel_launches = plan.kernel_time()
     plan.time_kernel(0)
     y_host = y.cpu().numpy()
+    errors = {"slab_plan": fp64_error(y_host, exact_y, 1e-12)}
     ms_merge, _ = timer.run(step(s_merge), max(10, args.steps // 2), 3)
     y_merge = y.cpu().numpy()
+    errors["merge_path"] = fp64_error(y_merge, exact_y, 1e-12)
     ms_default, _ = timer.run(step(s_default), max(5, args.steps // 10), 2)
+    errors["default_thread_per_row"] = fp64_error(y.cpu().numpy(), exact_y, 1e-12)
     learned = None
     try:
         from taco_amd import selector as S
         sel = S.Selector()
         c, s_learned, pred_ms = sel.choose(a, 0)
         ms_learned, _ = timer.run(step(s_learned), max(10, args.steps // 2), 3)
+        errors["learned_" + S.CLASS_NAMES[c]] = fp64_error(y.cpu().numpy(), exact_y, 1e-12)
         learned = {"class": S.CLASS_NAMES[c], "ms": ms_learned, "predicted_ms": pred_ms,
                    "speedup_vs_default": ms_default / ms_learned}
     except (OSError, ValueError) as e:  # no trained weights in this tree
@@ -253,9 +306,11 @@ def bench_spmv(args, A, x, name, workload_desc):
     gbs = nbytes / (ms * 1e-3) / 1e9
     check = None
     if y_ref is not None:
+        # taco's CPU result against the FP64 oracle, and the GPU's against it
         scale = max(np.abs(y_ref).max(), 1e-300)
-        check = {"slab_plan": float(np.abs(y_host - y_ref).max() / scale),
-                 "merge_path": float(np.abs(y_merge - y_ref).max() / scale)}
+        errors["taco_cpu"] = fp64_error(y_ref, exact_y, 1e-12)
+        check = {"slab_plan_vs_taco_cpu_normwise": float(np.abs(y_host - y_ref).max() / scale)}
+    del exact_y
     cpu = None
     if cpu_ts:
         cms = trimmed_mean(cpu_ts)
@@ -265,6 +320,16 @@ def bench_spmv(args, A, x, name, workload_desc):
                          f"(10% trimmed mean), JIT excluded, inputs in host DRAM"}
     traffic = profiled_traffic("k_slab_spmv_wave", name)
     kernel_gbs = nbytes / (kernel_ms * 1e-3) / 1e9
+    spmm_legs = None
+    if spmm is not None:
+        B, exact_c, exact_bf16, cpu_spmm = spmm
+        del plan
+        torch.cuda.empty_cache()
+        spmm_legs = {}
+        for dt in args.spmm_dtypes.split(","):
+            log(f"gpu: SpMM {dt} K={B.shape[1]} ...")
+            spmm_legs[dt] = spmm_leg(args, A, dt, pos, crd, B, exact_c, exact_bf16, timer,
+                                     cpu_spmm if dt == "f64" else None)
     return {
         "metric": "SpMV HBM GB/s (% of 8 TB/s), SpMM GFLOP/s: learned vs default schedule at tol",
         "value": gbs, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -272,8 +337,8 @@ def bench_spmv(args, A, x, name, workload_desc):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seed-generated)",
         "config": {"workload": name, "desc": workload_desc, "m": A.m, "n": A.n,
-                   "nnz": A.nnz, "schedule": f"slab-tiled plan parts={plan.info()['parts']} "
-                                             f"slab_width={plan.info()['slab_width']} unroll={args.unroll or 1} "
+                   "nnz": A.nnz, "schedule": f"slab-tiled plan parts={plan_info['parts']} "
+                                             f"slab_width={plan_info['slab_width']} unroll={args.unroll or 1} "
                                              f"threads=256 wave_reduce=1",
                    "parallelism": "single GPU"},
         "pct_of_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS,
@@ -291,29 +356,40 @@ def bench_spmv(args, A, x, name, workload_desc):
                      "kernel": "k_slab_spmv_wave", "kernel_us": kernel_ms * 1e3,
                      "kernel_launches_timed": kernel_launches,
                      "algorithmic_bytes_per_launch": nbytes},
+        "error_vs_fp64_oracle": errors,
         "check_vs_cpu_reference": check,
         "cpu_baseline": cpu,
+        "spmm_k64": spmm_legs,
     }
 
 
-def bench_spmm(args, A, dtype):
-    from taco_amd import workloads as W
-    K = 64
-    B = W.uniform(A.n * K, 2, W.STREAM_B).reshape(A.n, K)
-    C_ref, cpu_ts, cores, kind = (None, [], 0, None)
-    if not args.no_cpu and dtype == "f64":
-        log("cpu baseline: taco SpMM ...")
-        C_ref, cpu_ts, cores, kind = cpu_baseline_spmm(A, B, max(2, args.cpu_reps // 3))
-    torch, L, _lib = gpu_setup()
-    from taco_amd import descriptors as D
-    pos = torch.from_numpy(A.pos).cuda()
-    crd = torch.from_numpy(A.crd).cuda()
+def fp64_error(got, exact_pair, tol):
+    """Condition-scaled error of a result against the FP64 oracle (gate
+    |got - exact| <= tol * sum|a*b|, plus max elementwise-relative and
+    normwise errors); computed outside every timed region."""
+    import oracle
+    exact, cond = exact_pair
+    return oracle.error_report(got, exact, cond, tol)
+
+
+SPMM_TOL = {"f64": 1e-12, "f32": 1e-5, "bf16": 1e-2}
+
+
+def spmm_leg(args, A, dtype, pos, crd, B, exact, exact_bf16, timer, cpu=None):
+    """SpMM C = A B with K = 64 in one precision form: the library's main
+    schedule (merge path; MFMA for bf16), the learned selector's pick and
+    taco's default schedule (one thread per row), each timed with HIP
+    events; GFLOP/s, rooflines (algorithmic and gather-inclusive bytes) and
+    the error of every result against the FP64 oracle."""
+    import torch
+    from taco_amd import _lib, descriptors as D, workloads as W
+    L = _lib.lib()
+    K = B.shape[1]
     if dtype == "bf16":
         vals = torch.from_numpy(A.vals.astype(np.float32)).to(torch.bfloat16).view(torch.int16).cuda()
         Bd = torch.from_numpy(B.astype(np.float32)).to(torch.bfloat16).view(torch.int16).cuda()
         C = torch.empty((A.m, K), dtype=torch.float32, device="cuda")
-        cdt = "f32"
-        sizes = (2, 2, 4)
+        cdt, sizes = "f32", (2, 2, 4)
     else:
         tdt = torch.float64 if dtype == "f64" else torch.float32
         vals = torch.from_numpy(A.vals).to(tdt).cuda()
@@ -323,45 +399,117 @@ def bench_spmm(args, A, dtype):
         sizes = (8, 8, 8) if dtype == "f64" else (4, 4, 4)
     a = D.csr(pos, crd, vals, A.shape, dtype)
     bdsc, cdsc = D.dense(Bd, dtype), D.dense(C, cdt)
-    s_main = sched(_lib, _lib.MFMA) if dtype == "bf16" and args.mfma else \
+    main_name = "mfma" if dtype == "bf16" and args.mfma else f"merge{args.spmm_items}"
+    s_main = sched(_lib, _lib.MFMA) if main_name == "mfma" else \
         sched(_lib, _lib.MERGE_PATH, 256, args.spmm_items, 64)
     s_default = sched(_lib, _lib.THREAD_PER_ROW, 256)
 
-    def step(s):
-        return lambda: _lib.check(L.taco_hip_spmm_compute(cdsc.ptr, a.ptr, bdsc.ptr, s))
+    def step(s_):
+        return lambda: _lib.check(L.taco_hip_spmm_compute(cdsc.ptr, a.ptr, bdsc.ptr, s_))
 
-    timer = Timer(torch)
+    def result():
+        torch.cuda.synchronize()
+        return C.cpu().numpy().astype(np.float64)
+
+    tol = SPMM_TOL[dtype]
     ms, wall_ms = timer.run(step(s_main), args.steps, args.warmup)
-    C_host = C.cpu().numpy().astype(np.float64)
+    errors = {main_name: fp64_error(result(), exact, tol)}
+    if dtype == "bf16":
+        errors[main_name + "_vs_bf16_rounded_inputs"] = fp64_error(result(), exact_bf16, 1e-5)
+    learned = None
+    try:
+        from taco_amd import selector as S
+        c, s_l, pred_ms = S.Selector().choose(a, 1, K)
+        ms_l, _ = timer.run(step(s_l), max(5, args.steps // 2), 2)
+        errors["learned_" + S.CLASS_NAMES[c]] = fp64_error(result(), exact, tol)
+        learned = {"class": S.CLASS_NAMES[c], "ms": ms_l, "predicted_ms": pred_ms,
+                   "gflops": 2.0 * A.nnz * K / (ms_l * 1e-3) / 1e9}
+    except (OSError, ValueError) as e:  # no trained weights in this tree
+        learned = {"error": str(e)}
     ms_default, _ = timer.run(step(s_default), 2, 1)
+    errors["default_thread_per_row"] = fp64_error(result(), exact, tol)
+    if learned and "ms" in learned:
+        learned["speedup_vs_default"] = ms_default / learned["ms"]
     nbytes = W.algorithmic_bytes_spmm(A.m, A.n, A.nnz, K, *sizes)
+    gather_bytes = nbytes - sizes[1] * A.n * K + sizes[1] * A.nnz * K
     flops = 2.0 * A.nnz * K
-    gflops = flops / (ms * 1e-3) / 1e9
     gbs = nbytes / (ms * 1e-3) / 1e9
-    check = None
-    if C_ref is not None:
-        check = float(np.abs(C_host - C_ref).max() / max(np.abs(C_ref).max(), 1e-300))
-    cpu = None
-    if cpu_ts:
-        cms = trimmed_mean(cpu_ts)
-        cpu = {"value": flops / (cms * 1e-3) / 1e9, "unit": "GFLOP/s", "cores": cores, "host": host_cpu(),
-               "kind": kind, "ms": cms, "sample": f"whole workload, {len(cpu_ts)} calls"}
-    return {
-        "metric": "SpMM GFLOP/s", "value": gflops, "unit": "GFLOP/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
-        "wall_ms_per_step": wall_ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": dtype if dtype != "bf16" else "bf16 (f32 accumulate)",
-        "data": "synthetic (seed-generated)",
-        "config": {"workload": f"spmm_powerlaw_2m_k64_{dtype}", "m": A.m, "nnz": A.nnz,
-                   "k": K, "schedule": "mfma" if (dtype == "bf16" and args.mfma) else "merge_path"},
-        "default_schedule": {"ms": ms_default, "speedup_vs_default": ms_default / ms},
+    out = {
+        "dtype": dtype if dtype != "bf16" else "bf16 (f32 accumulate)",
+        "schedule": main_name, "ms": ms, "wall_ms": wall_ms,
+        "gflops": flops / (ms * 1e-3) / 1e9,
+        "learned_schedule": learned,
+        "default_schedule": {"schedule": "thread_per_row (taco split(i,256))", "ms": ms_default,
+                             "speedup_vs_default": ms_default / ms},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes": nbytes,
-                     "gather_inclusive_bytes": nbytes - sizes[1] * A.n * K + sizes[1] * A.nnz * K},
-        "check_vs_cpu_reference": check,
-        "cpu_baseline": cpu,
+                     "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": nbytes,
+                     "gather_inclusive_bytes": gather_bytes,
+                     "gather_inclusive_gbs": gather_bytes / (ms * 1e-3) / 1e9},
+        "error_vs_fp64_oracle": errors,
     }
+    if dtype == "bf16":
+        # MFMA utilisation: flops the MFMA tiles issue vs the dense bf16 peak.
+        out["mfma"] = {"useful_tflops": flops / (ms * 1e-3) / 1e12,
+                       "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                       "frac": flops / (ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS}
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    del a, bdsc, cdsc, vals, Bd, C
+    torch.cuda.empty_cache()
+    return out
+
+
+def spmm_inputs(args, A, dtypes):
+    """B (2,000,000 x 64, U[-1,1]), the FP64 oracle on it (and on its bf16
+    rounding when bf16 is measured) and, for f64, taco's CPU SpMM timed on
+    the host cores (a bounded sample of calls)."""
+    import oracle
+    from taco_amd import workloads as W
+    K = SPMM_K
+    B = W.uniform(A.n * K, 2, W.STREAM_B).reshape(A.n, K)
+    cpu = None
+    if not args.no_cpu and "f64" in dtypes:
+        reps = max(2, args.cpu_reps // 6)
+        log(f"cpu baseline: taco SpMM on {host_cores()} cores, {reps} calls ...")
+        C_ref, cpu_ts, cores, kind = cpu_baseline_spmm(A, B, reps)
+        cms = trimmed_mean(cpu_ts)
+        cpu = {"value": 2.0 * A.nnz * K / (cms * 1e-3) / 1e9, "unit": "GFLOP/s", "cores": cores,
+               "kind": kind, "ms": cms, "median_ms": float(np.median(cpu_ts)), "host": host_cpu(),
+               "sample": f"whole workload, {len(cpu_ts)} timed compute() calls (10% trimmed mean)"}
+        del C_ref
+    oracle.set_threads(host_cores())
+    log("fp64 oracle for SpMM ...")
+    exact = oracle.exact_spmm(A, B)
+    exact_bf16 = None
+    if "bf16" in dtypes:
+        import torch
+        r = lambda v: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float32)) \
+            .to(torch.bfloat16).to(torch.float64).numpy()
+        from taco_amd import workloads as W2
+        Ar = W2.Csr(A.m, A.n, A.pos, A.crd, r(A.vals))
+        exact_bf16 = oracle.exact_spmm(Ar, r(B))
+    return B, exact, exact_bf16, cpu
+
+
+def bench_spmm(args, A, dtype):
+    """One SpMM precision form on its own JSON line (--workload spmm_<dtype>)."""
+    B, exact, exact_bf16, cpu = spmm_inputs(args, A, (dtype,))
+    torch, L, _lib = gpu_setup()
+    pos = torch.from_numpy(A.pos).cuda()
+    crd = torch.from_numpy(A.crd).cuda()
+    leg = spmm_leg(args, A, dtype, pos, crd, B, exact, exact_bf16, Timer(torch),
+                   cpu if dtype == "f64" else None)
+    out = {
+        "metric": "SpMM GFLOP/s", "value": leg["gflops"], "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": leg["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": leg["dtype"], "data": "synthetic (seed-generated)",
+        "config": {"workload": f"spmm_powerlaw_2m_k64_{dtype}", "m": A.m, "nnz": A.nnz,
+                   "k": SPMM_K, "schedule": leg["schedule"]},
+        "roofline": leg["roofline"], "cpu_baseline": leg.get("cpu_baseline"),
+    }
+    out.update({k: v for k, v in leg.items() if k not in out})
+    return out
 
 
 def bench_spgemm(args):
@@ -655,6 +803,8 @@ def main():
     ap.add_argument("--unroll", type=int, default=0, help="slab plan entries per thread per step (0 = library default)")
     ap.add_argument("--spmm-items", type=int, default=256)
     ap.add_argument("--mfma", type=int, default=1)
+    ap.add_argument("--spmm-dtypes", default="f64,f32,bf16",
+                    help="SpMM legs of the default workload (empty: SpMV only)")
     ap.add_argument("--cpu-reps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--instances", type=int, default=4096, help="batched workloads")
@@ -675,8 +825,12 @@ def main():
         log(f"  nnz={A.nnz} in {time.perf_counter() - t0:.1f}s")
         if w == "spmv_powerlaw":
             x = W.uniform(A.n, 2, W.STREAM_X)
+            spmm = None
+            if args.spmm_dtypes:
+                spmm = spmm_inputs(args, A, args.spmm_dtypes.split(","))
             out = bench_spmv(args, A, x, "spmv_powerlaw_2m_f64",
-                             "2Mx2M CSR, Zipf(2) rows on [1,1e5] rescaled to mean 16")
+                             "2Mx2M CSR, Zipf(2) rows on [1,1e5] rescaled to mean 16",
+                             spmm=spmm)
         else:
             out = bench_spmm(args, A, w.split("_")[1])
     elif w == "spmv_uniform":

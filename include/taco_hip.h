@@ -43,12 +43,20 @@ void taco_hip_set_unified_memory_enabled(int enabled);
  * current device's architecture.  Returned string is owned by the library. */
 const char* taco_hip_get_default_compiler_flags(void);
 /* cuda_unified_alloc (cuda.cpp:54-64): NULL for size 0; managed memory when
- * unified memory is enabled, device memory otherwise. */
+ * unified memory is enabled, device memory otherwise.  On a machine without
+ * a GPU (code generation only) it returns host memory. */
 void* taco_hip_unified_alloc(size_t size);
 /* cuda_unified_free (cuda.cpp:66-72).  Frees memory from
  * taco_hip_unified_alloc or any allocation this library hands out (outputs
  * of assemble functions). */
 void taco_hip_unified_free(void* ptr);
+/* Reallocation for generated host code (taco's realloc emulation,
+ * codegen_cuda.cpp:1086-1136): a new taco_hip_unified_alloc block holding the
+ * first min(old, new) bytes of ptr, which is released.  NULL on failure. */
+void* taco_hip_unified_realloc(void* ptr, size_t old_bytes, size_t new_bytes);
+/* Zero-fill of such a block (Allocate with clear, codegen_cuda.cpp:1100-1112),
+ * complete when the call returns. */
+int taco_hip_unified_clear(void* ptr, size_t bytes);
 
 /* Description of the last failure on the calling thread. */
 const char* taco_hip_last_error(void);

@@ -19,6 +19,8 @@ extern "C" {
 void* taco_hip_get_stream(void);
 void* taco_hip_unified_alloc(size_t size);
 void taco_hip_unified_free(void* ptr);
+void* taco_hip_unified_realloc(void* ptr, size_t old_bytes, size_t new_bytes);
+int taco_hip_unified_clear(void* ptr, size_t bytes);
 int taco_hip_module_finish(int rc);
 int taco_hip_module_report(int code, const char* what);
 int taco_hip_host_read_bytes(void* dst, const void* src, size_t bytes);
@@ -169,12 +171,39 @@ __device__ inline T taco_hip_block_sum(T v) {
   return t;
 }
 
+/* Float atomic adds.  By default the hardware float atomics, valid on the
+ * coarse-grained device memory of hipMalloc.  Modules compiled with unified
+ * memory on (TACO_HIP_COHERENT_ATOMICS, set by the runtime's default flags)
+ * may write managed memory, where those may not take effect: they use an
+ * integer compare-and-swap loop instead. */
+#ifdef TACO_HIP_COHERENT_ATOMICS
+__device__ inline void taco_hip_atomicAdd(double* a, int64_t i, double v) {
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(a + i);
+  unsigned long long old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long assumed;
+  do {
+    assumed = old;
+    old = atomicCAS(p, assumed,
+                    (unsigned long long)__double_as_longlong(__longlong_as_double((long long)assumed) + v));
+  } while (assumed != old);
+}
+__device__ inline void taco_hip_atomicAdd(float* a, int64_t i, float v) {
+  unsigned int* p = reinterpret_cast<unsigned int*>(a + i);
+  unsigned int old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned int assumed;
+  do {
+    assumed = old;
+    old = atomicCAS(p, assumed, __float_as_uint(__uint_as_float(assumed) + v));
+  } while (assumed != old);
+}
+#else
 __device__ inline void taco_hip_atomicAdd(double* a, int64_t i, double v) {
   unsafeAtomicAdd(a + i, v);
 }
 __device__ inline void taco_hip_atomicAdd(float* a, int64_t i, float v) {
   unsafeAtomicAdd(a + i, v);
 }
+#endif
 __device__ inline void taco_hip_atomicAdd(int32_t* a, int64_t i, int32_t v) {
   atomicAdd(a + i, v);
 }
@@ -195,6 +224,130 @@ __device__ inline void taco_hip_atomicAddWave(T* a, int64_t i, T v) {
   } else {
     taco_hip_atomicAdd(a, i, v);
   }
+}
+
+/* Atomic multiply (taco's `*=` under Atomics, codegen_cuda.cpp:1354-1378,
+ * where a single compare-and-swap is issued without a retry): a
+ * compare-and-swap loop that retries until the product lands. */
+__device__ inline void taco_hip_atomicMul(double* a, int64_t i, double v) {
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(a + i);
+  unsigned long long old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long assumed;
+  do {
+    assumed = old;
+    old = atomicCAS(p, assumed,
+                    (unsigned long long)__double_as_longlong(__longlong_as_double((long long)assumed) * v));
+  } while (assumed != old);
+}
+__device__ inline void taco_hip_atomicMul(float* a, int64_t i, float v) {
+  unsigned int* p = reinterpret_cast<unsigned int*>(a + i);
+  unsigned int old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned int assumed;
+  do {
+    assumed = old;
+    old = atomicCAS(p, assumed, __float_as_uint(__uint_as_float(assumed) * v));
+  } while (assumed != old);
+}
+__device__ inline void taco_hip_atomicMul(int32_t* a, int64_t i, int32_t v) {
+  int32_t old = __hip_atomic_load(a + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int32_t assumed;
+  do {
+    assumed = old;
+    old = atomicCAS(a + i, assumed, assumed * v);
+  } while (assumed != old);
+}
+
+/* Orders the LDS accesses of the lanes of one wave around a group
+ * reduction: the group lies inside one wave, whose lanes execute together,
+ * so a compiler and wavefront-scope memory fence is all it needs. */
+__device__ inline void taco_hip_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* A group reduction's barrier when the group width is known only at run
+ * time: wave-level ordering for groups inside one wave, else the
+ * workgroup barrier. */
+__device__ inline void taco_hip_group_sync(int32_t width) {
+  if (width <= 64 && 64 % width == 0)
+    taco_hip_wave_sync();
+  else
+    __syncthreads();
+}
+
+/* ---- host side of modules printed from taco's own IR ---- */
+
+/* The host waits for the library stream before code that may touch memory
+ * a queued kernel uses. */
+static inline int taco_hip_stream_wait() {
+  return (int)hipStreamSynchronize(taco_hip_stream());
+}
+
+/* A host scalar's device copy for a kernel that assigns it, and its value
+ * back after the launch (stream-ordered, then waited for). */
+static inline int taco_hip_copy_in(void* dev, const void* host, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, taco_hip_stream());
+  if (e != hipSuccess) return (int)e;
+  return (int)hipStreamSynchronize(taco_hip_stream());
+}
+static inline int taco_hip_copy_out(void* host, const void* dev, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, taco_hip_stream());
+  if (e != hipSuccess) return (int)e;
+  return (int)hipStreamSynchronize(taco_hip_stream());
+}
+
+/* cudaMemset of taco's initValues (lowerer_impl_imperative.cpp:3357-3362):
+ * stream-ordered. */
+static inline int taco_hip_memsetAsync(void* p, int value, size_t bytes) {
+  if (bytes == 0) return 0;
+  return (int)hipMemsetAsync(p, value, bytes, taco_hip_stream());
+}
+
+static inline int taco_hip_cmp_i32(const void* a, const void* b) {
+  const int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+  return (x > y) - (x < y);
+}
+
+/* Row of each workgroup's first entry (taco_binarySearchBeforeBlockLaunch,
+ * codegen_cuda.cpp:112-122): results[b] = binarySearchBefore(array, lo, hi,
+ * b * per_block) for b in [0, num_blocks].  Launched on the library stream;
+ * returns results, which taco's IR assigns back. */
+static inline int32_t* taco_hip_searchBlockStarts(const int32_t* array,
+                                                  int32_t* results, int32_t lo,
+                                                  int32_t hi, int32_t per_block,
+                                                  int32_t block_size,
+                                                  int32_t num_blocks) {
+  (void)block_size;
+  if (num_blocks < 0) return results;
+  int64_t grid = ((int64_t)num_blocks + 1 + 255) / 256;
+  hipLaunchKernelGGL(taco_hip_binarySearchBeforeBlock, dim3((unsigned)grid),
+                     dim3(256), 0, taco_hip_stream(), array, results, lo, hi,
+                     per_block, num_blocks);
+  return results;
+}
+
+/* results[b] = binarySearchBefore(array, lo, hi, targets[b]) for b in
+ * [0, num_blocks] (codegen_cuda.cpp:124-134). */
+__global__ __launch_bounds__(256) void taco_hip_binarySearchIndirectBeforeBlock(
+    const int32_t* __restrict__ array, int32_t* __restrict__ results,
+    int32_t lo, int32_t hi, const int32_t* __restrict__ targets,
+    int32_t num_blocks) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > num_blocks) return;
+  results[b] = taco_hip_binarySearchBefore(array, lo, hi, targets[b]);
+}
+
+static inline int32_t* taco_hip_searchBlockStartsIndirect(
+    const int32_t* array, int32_t* results, int32_t lo, int32_t hi,
+    const int32_t* targets, int32_t block_size, int32_t num_blocks) {
+  (void)block_size;
+  if (num_blocks < 0) return results;
+  int64_t grid = ((int64_t)num_blocks + 1 + 255) / 256;
+  hipLaunchKernelGGL(taco_hip_binarySearchIndirectBeforeBlock,
+                     dim3((unsigned)grid), dim3(256), 0, taco_hip_stream(),
+                     array, results, lo, hi, targets, num_blocks);
+  return results;
 }
 
 /* ---- fills (a kernel rather than a memset node, safe under capture) ---- */

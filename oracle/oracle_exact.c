@@ -3,15 +3,17 @@
  *
  * Every output element is summed in long double (x87 80-bit, 64-bit
  * mantissa) with Neumaier's compensated summation, then rounded once to
- * double.  Each product a*b of two doubles is exact in long double only to
- * 64 of its 106 bits, so the product is additionally split with the
- * error-free two-product via fmal: p = a*b rounded, e = fma(a, b, -p) exact.
- * The result is accurate to well below one double ulp of
- * sum_j |a_ij b_j| for all sizes used here, which is what the tolerance gate
- * |got - exact| <= tol * sum_j |a_ij b_j| needs.
+ * double.  Each product a*b of two doubles is first split exactly into two
+ * doubles by the FMA two-product (Ogita, Rump and Oishi, "Accurate sum and
+ * dot product", 2005): p = fl(a*b), e = fma(a, b, -p), with p + e = a*b
+ * exactly (no underflow occurs for the value ranges used here); both terms
+ * enter the compensated sum.  The result is accurate to well below one
+ * double ulp of sum_j |a_ij b_j| for all sizes used here, which is what the
+ * tolerance gate |got - exact| <= tol * sum_j |a_ij b_j| needs.
  *
- * Compiled without -ffast-math (see oracle/Makefile): fast-math would allow
- * the compiler to cancel the compensation terms algebraically.
+ * Compiled without -ffast-math and in ISO C mode, so without floating-point
+ * contraction (see oracle/Makefile): fast-math would allow the compiler to
+ * cancel the compensation terms algebraically.
  */
 #include "taco_cpu.h"
 
@@ -31,10 +33,10 @@ static inline void nadd(nsum_t* a, long double v) {
   a->s = t;
 }
 
-/* Add the exact product a*b: two long-double terms whose sum is exact. */
+/* Add the exact product a*b as its two-product terms p + e. */
 static inline void nadd_prod(nsum_t* acc, double a, double b) {
-  long double p = (long double)a * (long double)b;
-  long double e = fmal((long double)a, (long double)b, -p);
+  double p = a * b;
+  double e = fma(a, b, -p);
   nadd(acc, p);
   nadd(acc, e);
 }
@@ -58,22 +60,38 @@ void exact_spmv(int m, const int32_t* pos, const int32_t* crd,
   }
 }
 
+/* Row by row: each nonzero's B row is read once and added into k running
+ * sums.  The terms of every output (i, kk) are added in entry order, as in
+ * exact_spmv, so column kk equals exact_spmv(A, B[:, kk]) bit for bit. */
 void exact_spmm(int m, int k, const int32_t* pos, const int32_t* crd,
                 const double* vals, const double* B, double* C_exact,
                 double* cond) {
-#pragma omp parallel for schedule(static)
-  for (int32_t i = 0; i < m; i++) {
-    for (int32_t kk = 0; kk < k; kk++) {
-      nsum_t acc = {0, 0};
-      long double c = 0;
-      for (int32_t p = pos[i]; p < pos[i + 1]; p++) {
-        double a = vals[p], b = B[(int64_t)crd[p] * k + kk];
-        nadd_prod(&acc, a, b);
-        c += fabsl((long double)a * (long double)b);
+#pragma omp parallel
+  {
+    nsum_t* acc = (nsum_t*)malloc(sizeof(nsum_t) * (size_t)(k > 0 ? k : 1));
+    long double* cs = (long double*)malloc(sizeof(long double) * (size_t)(k > 0 ? k : 1));
+#pragma omp for schedule(dynamic, 256)
+    for (int32_t i = 0; i < m; i++) {
+      for (int32_t kk = 0; kk < k; kk++) {
+        acc[kk].s = 0;
+        acc[kk].c = 0;
+        cs[kk] = 0;
       }
-      if (C_exact) C_exact[(int64_t)i * k + kk] = nresult(&acc);
-      if (cond) cond[(int64_t)i * k + kk] = (double)c;
+      for (int32_t p = pos[i]; p < pos[i + 1]; p++) {
+        const double a = vals[p];
+        const double* brow = B + (int64_t)crd[p] * k;
+        for (int32_t kk = 0; kk < k; kk++) {
+          nadd_prod(&acc[kk], a, brow[kk]);
+          cs[kk] += fabsl((long double)a * (long double)brow[kk]);
+        }
+      }
+      for (int32_t kk = 0; kk < k; kk++) {
+        if (C_exact) C_exact[(int64_t)i * k + kk] = nresult(&acc[kk]);
+        if (cond) cond[(int64_t)i * k + kk] = (double)cs[kk];
+      }
     }
+    free(acc);
+    free(cs);
   }
 }
 

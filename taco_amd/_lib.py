@@ -57,6 +57,8 @@ _SIGS = {
     "taco_hip_get_default_compiler_flags": (c_char_p, []),
     "taco_hip_unified_alloc": (c_void_p, [c_size_t]),
     "taco_hip_unified_free": (None, [c_void_p]),
+    "taco_hip_unified_realloc": (c_void_p, [c_void_p, c_size_t, c_size_t]),
+    "taco_hip_unified_clear": (c_int, [c_void_p, c_size_t]),
     "taco_hip_last_error": (c_char_p, []),
     "taco_hip_device_count": (c_int, []),
     "taco_hip_device_arch": (c_char_p, []),

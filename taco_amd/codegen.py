@@ -27,6 +27,12 @@ STANDARD_SPECS = {
 }
 
 
+# Modules the unified-memory tests compile with unified memory on (their
+# flags, and so their cache entries, differ: compare-and-swap atomics).
+UNIFIED_PREBUILT = ("spmv_default_f64", "spmv_nnz_f64", "spmv_merge_f64",
+                    "spmm_nnz_f64", "spmm_merge_f64")
+
+
 def spec(op, family, csize=64, block_size=256, nnz_per_thread=0, warp_size=0,
          rows_per_warp=0, bound=0):
     return _lib.GenSpec(op, family, csize, block_size, nnz_per_thread,
@@ -60,8 +66,17 @@ class Kernel:
 
 
 def prebuild_cache(names=None):
-    """Compiles the standard modules into the JIT cache (no GPU needed)."""
+    """Compiles the standard modules into the JIT cache (no GPU needed),
+    and the UNIFIED_PREBUILT ones also as compiled under unified memory."""
     paths = {}
     for name in names or STANDARD_SPECS:
         paths[name] = Kernel(standard(name)).path
+    L = _lib.lib()
+    was = L.taco_hip_should_use_unified_memory()
+    L.taco_hip_set_unified_memory_enabled(1)
+    try:
+        for name in UNIFIED_PREBUILT:
+            paths[name + "@unified"] = Kernel(standard(name)).path
+    finally:
+        L.taco_hip_set_unified_memory_enabled(was)
     return paths

@@ -19,6 +19,11 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 
 hipStream_t current_stream();
 bool blocking();
+// True when hardware float atomics are valid on `p`: always with unified
+// memory off (entry points then take hipMalloc device memory), otherwise
+// only for plain device allocations.  Kernels that add into an output with
+// float atomics take the answer as their `coarse` argument.
+bool coarse_grained(const void* p);
 // Synchronise the stream when blocking mode is on; reports launch errors.
 int finish(const char* what);
 

@@ -66,13 +66,44 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// Hardware float atomic add (global_atomic_add_f64 / _f32): device memory
-// from hipMalloc is coarse-grained, where these are valid.
-__device__ __forceinline__ void atomic_add(double* p, double v) {
-  unsafeAtomicAdd(p, v);
+// Float atomic add by an integer compare-and-swap loop.  Valid on every kind
+// of memory a kernel can reach, including managed and fine-grained host
+// memory, where the hardware float atomics may not take effect.
+__device__ __forceinline__ void atomic_add_cas(double* p, double v) {
+  unsigned long long* a = reinterpret_cast<unsigned long long*>(p);
+  unsigned long long old = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long assumed;
+  do {
+    assumed = old;
+    old = atomicCAS(a, assumed,
+                    (unsigned long long)__double_as_longlong(__longlong_as_double((long long)assumed) + v));
+  } while (assumed != old);
 }
-__device__ __forceinline__ void atomic_add(float* p, float v) {
-  unsafeAtomicAdd(p, v);
+__device__ __forceinline__ void atomic_add_cas(float* p, float v) {
+  unsigned int* a = reinterpret_cast<unsigned int*>(p);
+  unsigned int old = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned int assumed;
+  do {
+    assumed = old;
+    old = atomicCAS(a, assumed, __float_as_uint(__uint_as_float(assumed) + v));
+  } while (assumed != old);
+}
+
+// Float atomic add on an output array.  coarse: the array is device memory
+// from hipMalloc (coarse-grained), where the hardware float atomics
+// (global_atomic_add_f64 / _f32) are valid; otherwise (managed memory) the
+// compare-and-swap form is used.  `coarse` is uniform over the launch.
+__device__ __forceinline__ void atomic_add(double* p, double v, bool coarse) {
+  if (coarse)
+    unsafeAtomicAdd(p, v);
+  else
+    atomic_add_cas(p, v);
+}
+__device__ __forceinline__ void atomic_add(float* p, float v, bool coarse) {
+  if (coarse)
+    unsafeAtomicAdd(p, v);
+  else
+    atomic_add_cas(p, v);
 }
 
 }  // namespace taco_hip

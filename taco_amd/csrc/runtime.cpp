@@ -72,6 +72,16 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 hipStream_t current_stream() { return t_stream; }
 bool blocking() { return t_blocking; }
 
+bool coarse_grained(const void* p) {
+  if (!g_unified_memory || !p) return true;
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice && !attr.isManaged;
+}
+
 int finish(const char* what) {
   TH_LAUNCH_CHECK(what);
   if (t_blocking) TH_CHECK(hipStreamSynchronize(t_stream));
@@ -210,15 +220,29 @@ const char* taco_hip_get_default_compiler_flags(void) {
   // Counterpart of nvcc's "-w -O3 -Xcompiler -fPIC -shared -ffast-math -O3
   // --generate-code arch=compute_XY,code=sm_XY" (cuda.cpp:30-40).  Packed
   // FP32 instructions are disabled: hipcc forms them from plain float code.
+  // With unified memory on, kernels may write managed memory, where the
+  // hardware float atomics may not take effect: generated atomics then use
+  // compare-and-swap loops (TACO_HIP_COHERENT_ATOMICS).
   g_flags = std::string("-w -O3 -fPIC -shared -std=c++17 --offload-arch=") +
             taco_hip_device_arch() +
-            " -Xclang -target-feature -Xclang -packed-fp32-ops"
-            " -munsafe-fp-atomics";
+            " -Xclang -target-feature -Xclang -packed-fp32-ops" +
+            (g_unified_memory ? " -DTACO_HIP_COHERENT_ATOMICS"
+                              : " -munsafe-fp-atomics");
   return g_flags.c_str();
+}
+
+// Without any GPU (code generation only: modules are printed and compiled,
+// nothing can be launched), allocations are host memory, so that taco's
+// host-side packing code still runs.
+static bool has_device() {
+  static int n = -1;
+  if (n < 0) n = taco_hip_device_count();
+  return n > 0;
 }
 
 void* taco_hip_unified_alloc(size_t size) {
   if (size == 0) return nullptr;
+  if (!has_device()) return std::malloc(size);
   void* p = nullptr;
   hipError_t e = g_unified_memory ? hipMallocManaged(&p, size)
                                   : hipMalloc(&p, size);
@@ -231,8 +255,41 @@ void* taco_hip_unified_alloc(size_t size) {
 
 void taco_hip_unified_free(void* ptr) {
   if (!ptr) return;
+  if (!has_device()) {
+    std::free(ptr);
+    return;
+  }
   hipError_t e = hipFree(ptr);
   if (e != hipSuccess) hip_fail(e, "taco_hip_unified_free", __FILE__, __LINE__);
+}
+
+void* taco_hip_unified_realloc(void* ptr, size_t old_bytes, size_t new_bytes) {
+  void* q = taco_hip_unified_alloc(new_bytes);
+  if (!q && new_bytes > 0) return nullptr;
+  const size_t n = old_bytes < new_bytes ? old_bytes : new_bytes;
+  if (ptr && n > 0) {
+    if (!has_device()) {
+      std::memcpy(q, ptr, n);
+    } else if (hipMemcpyAsync(q, ptr, n, hipMemcpyDefault, t_stream) != hipSuccess ||
+               hipStreamSynchronize(t_stream) != hipSuccess) {
+      hip_fail(hipGetLastError(), "taco_hip_unified_realloc", __FILE__, __LINE__);
+      taco_hip_unified_free(q);
+      return nullptr;
+    }
+  }
+  taco_hip_unified_free(ptr);
+  return q;
+}
+
+int taco_hip_unified_clear(void* ptr, size_t bytes) {
+  if (!ptr || bytes == 0) return 0;
+  if (!has_device()) {
+    std::memset(ptr, 0, bytes);
+    return 0;
+  }
+  TH_CHECK(hipMemsetAsync(ptr, 0, bytes, t_stream));
+  TH_CHECK(hipStreamSynchronize(t_stream));
+  return 0;
 }
 
 const char* taco_hip_last_error(void) { return t_last_error.c_str(); }

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void k_spmm_nnz_split(
     int32_t m, int64_t nnz, int32_t K, int32_t nnz_per_team,
     const int32_t* __restrict__ pos, const int32_t* __restrict__ crd,
     const TA* __restrict__ vals, const TA* __restrict__ B,
-    TC* __restrict__ C) {
+    TC* __restrict__ C, bool coarse) {
   const int lane = threadIdx.x & (G - 1);
   const int64_t team = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const int64_t z0 = team * nnz_per_team;
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_spmm_nnz_split(
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
         const int k = lane + c * G;
-        if (k < K) atomic_add(C + (int64_t)r * K + k, acc[c]);
+        if (k < K) atomic_add(C + (int64_t)r * K + k, acc[c], coarse);
         acc[c] = 0;
       }
       do {
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_spmm_nnz_split(
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
     const int k = lane + c * G;
-    if (k < K) atomic_add(C + (int64_t)r * K + k, acc[c]);
+    if (k < K) atomic_add(C + (int64_t)r * K + k, acc[c], coarse);
   }
 }
 
@@ -380,7 +380,8 @@ int run_spmm(const CsrView& a, int32_t K, const TA* B, TC* C,
 #define TH_X(g, c)                                                          \
   if (G == g && CPL == c) {                                                 \
     k_spmm_nnz_split<TA, TC, g, c><<<grid, bs, 0, st>>>(                    \
-        a.m, a.nnz, K, npt, a.pos, a.crd, (const TA*)a.vals, B, C);         \
+        a.m, a.nnz, K, npt, a.pos, a.crd, (const TA*)a.vals, B, C,          \
+        coarse_grained(C));                                                 \
     return 0;                                                               \
   }
     TH_SHAPES(TH_X)

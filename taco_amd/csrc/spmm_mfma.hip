@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     int32_t m, int32_t K, int32_t ngroups, const int32_t* __restrict__ part_pos,
     const int32_t* __restrict__ pos, const int32_t* __restrict__ crd,
     const uint16_t* __restrict__ vals, const uint16_t* __restrict__ B,
-    float* __restrict__ C) {
+    float* __restrict__ C, bool coarse) {
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   const int64_t t = (int64_t)blockIdx.x * 4 + wave;
@@ -147,10 +147,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
         const v4f32 o = {acc[0][r], acc[1][r], acc[2][r], acc[3][r]};
         __builtin_nontemporal_store(o, (v4f32*)dst);
       } else {
-        atomic_add(dst + 0, acc[0][r]);
-        atomic_add(dst + 1, acc[1][r]);
-        atomic_add(dst + 2, acc[2][r]);
-        atomic_add(dst + 3, acc[3][r]);
+        atomic_add(dst + 0, acc[0][r], coarse);
+        atomic_add(dst + 1, acc[1][r], coarse);
+        atomic_add(dst + 2, acc[2][r], coarse);
+        atomic_add(dst + 3, acc[3][r], coarse);
       }
     }
   }
@@ -181,7 +181,8 @@ int spmm_mfma_bf16(const CsrView& a, int32_t K, const void* B, float* C,
   const unsigned grid = (unsigned)((max_tasks + 3) / 4);
   k_spmm_mfma_bf16<<<grid, 256, 0, st>>>(a.m, K, ngroups, part_pos, a.pos,
                                          a.crd, (const uint16_t*)a.vals,
-                                         (const uint16_t*)B, C);
+                                         (const uint16_t*)B, C,
+                                         coarse_grained(C));
   TH_LAUNCH_CHECK("k_spmm_mfma_bf16");
   return 0;
 }

@@ -84,7 +84,7 @@ template <typename T, int K>
 __global__ __launch_bounds__(1024) void k_spmv_nnz_split(
     int32_t m, int64_t nnz, const int32_t* __restrict__ pos,
     const int32_t* __restrict__ crd, const T* __restrict__ vals,
-    const T* __restrict__ x, T* __restrict__ y) {
+    const T* __restrict__ x, T* __restrict__ y, bool coarse) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t z0 = t * K;
   int32_t r = 0;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(1024) void k_spmv_nnz_split(
     int64_t rend = pos[r + 1];
     for (int64_t z = z0; z < z1; ++z) {
       while (z >= rend) {  // skips empty rows too
-        if (acc != T(0)) atomic_add(y + r, acc);
+        if (acc != T(0)) atomic_add(y + r, acc, coarse);
         acc = 0;
         ++r;
         rend = pos[r + 1];
@@ -112,9 +112,9 @@ __global__ __launch_bounds__(1024) void k_spmv_nnz_split(
   const bool same = !active || r == r_lead;
   if (__ballot(same) == ~0ull) {
     T s = wave_sum(active ? acc : T(0));
-    if ((int)__lane_id() == leader && act) atomic_add(y + r_lead, s);
+    if ((int)__lane_id() == leader && act) atomic_add(y + r_lead, s, coarse);
   } else if (active) {
-    atomic_add(y + r, acc);
+    atomic_add(y + r, acc, coarse);
   }
 }
 
@@ -325,10 +325,12 @@ int run_spmv(const CsrView& a, const T* x, T* y, const taco_hip_schedule& s) {
     if (a.nnz == 0) return 0;
     int64_t threads = (a.nnz + k - 1) / k;
     unsigned grid = (unsigned)((threads + bs - 1) / bs);
+    const bool coarse = coarse_grained(y);
 #define TH_NS(K)                                                           \
   case K:                                                                  \
     k_spmv_nnz_split<T, K><<<grid, bs, 0, st>>>(a.m, a.nnz, a.pos, a.crd,  \
-                                                (const T*)a.vals, x, y);   \
+                                                (const T*)a.vals, x, y,    \
+                                                coarse);                   \
     return 0;
     switch (k) {
       TH_NS(1) TH_NS(2) TH_NS(4) TH_NS(8) TH_NS(16) TH_NS(32)
