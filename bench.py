@@ -15,7 +15,10 @@ result.
 
 Other workloads (--workload): spmm_f64, spmm_f32, spmm_bf16 (same A, dense B
 2,000,000 x 64), spgemm (500,000^2, 8 entries per row), spmv_uniform (100,000^2,
-10 entries per row).
+10 entries per row), batched_spmv / batched_spmm (4,096 instances), selector,
+codegen (generated modules beside hand-written kernels), gather_ceiling (random
+8-byte load rate: the bound of the gather-limited SpMV kernels),
+copy_calibration (scales the rocprofv3 traffic counters).
 """
 import argparse
 import json
@@ -774,6 +777,59 @@ def bench_codegen(args):
     }
 
 
+# nnz of the 2M-row power-law matrix: one x gather per entry
+POWERLAW_GATHERS = 31_286_882
+
+
+def gather_ceiling(args):
+    """The rate at which MI355X serves random 8-byte loads with nothing else
+    in flight (taco_hip_gather_ceiling), swept over the footprint of the
+    gathered array, loads in flight per lane, workgroups per CU, and the
+    vector-L1 / scalar-cache load paths.  Each point reports the time the
+    power-law SpMV's 31.3M x gathers would take at that rate: the ceiling the
+    slab kernel (one gather per entry, 1 MB x slabs) is measured against."""
+    import ctypes
+    torch, L, _lib = gpu_setup()
+    out = (ctypes.c_double * 3)()
+    reps = max(3, min(args.steps, 20))
+    rows = []
+    for log2 in (11, 14, 17, 21, 25):
+        for mode, unrolls, wgs in ((0, (1, 4, 8, 16), (4, 8)), (1, (4, 16), (8,)),
+                                   (2, (4, 8), (8,))):
+            for u in unrolls:
+                for wg in wgs:
+                    log(f"gather ceiling: footprint 2^{log2} f64, mode {mode}, "
+                        f"unroll {u}, {wg} workgroups/CU")
+                    _lib.check(L.taco_hip_gather_ceiling(mode, u, log2, wg, POWERLAW_GATHERS,
+                                                         reps, out), "gather ceiling")
+                    ms, lane_loads, wave_loads = out[0], out[1], out[2]
+                    loads = lane_loads + wave_loads
+                    rate = loads / (ms * 1e-3)
+                    rows.append({"footprint_bytes": 8 << log2,
+                                 "path": ("vector", "scalar", "mixed")[mode],
+                                 "unroll": u, "wg_per_cu": wg, "ms": ms,
+                                 "lane_loads": lane_loads, "wave_loads": wave_loads,
+                                 "gloads_per_s": rate / 1e9,
+                                 "powerlaw_gathers_us": POWERLAW_GATHERS / rate * 1e6})
+    best = {}
+    for r in rows:
+        k = (r["footprint_bytes"], r["path"])
+        if k not in best or r["gloads_per_s"] > best[k]["gloads_per_s"]:
+            best[k] = r
+    l2 = best[(8 << 17, "vector")]
+    return {
+        "metric": "random 8-byte gather rate (G loads/s), 1 MB footprint, vector path",
+        "value": l2["gloads_per_s"], "unit": "G loads/s", "n_gpus": 1, "steps": reps,
+        "warmup": 1, "ms_per_step": l2["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (in-register index stream)",
+        "config": {"workload": "gather_ceiling", "gathers": POWERLAW_GATHERS},
+        "best": [{"footprint_bytes": k[0], "path": k[1], **{kk: v[kk] for kk in
+                  ("unroll", "wg_per_cu", "gloads_per_s", "powerlaw_gathers_us")}}
+                 for k, v in sorted(best.items())],
+        "sweep": rows, "roofline": None, "cpu_baseline": None,
+    }
+
+
 def copy_calibration(args):
     """A plain device copy of known size (2 GiB read + 2 GiB written per
     launch), run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE into
@@ -840,6 +896,8 @@ def main():
         out = bench_spgemm(args)
     elif w == "copy_calibration":
         out = copy_calibration(args)
+    elif w == "gather_ceiling":
+        out = gather_ceiling(args)
     elif w == "selector":
         out = bench_selector(args)
     elif w == "codegen":

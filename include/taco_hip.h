@@ -373,6 +373,21 @@ int taco_hip_module_report(int code, const char* what);
 int taco_hip_module_finish(int rc);
 int taco_hip_host_read_bytes(void* dst, const void* src, size_t bytes);
 
+/* ======================================================================
+ * Measurement.  No reference counterpart: a bound for the gather-limited
+ * SpMV kernels.  Times random 8-byte loads, and nothing else, from an f64
+ * array of 2^log2_entries entries (10..25).  mode 0: each lane loads its own
+ * element, `unroll` loads in flight per lane (1, 2, 4, 8, 16); mode 1: each
+ * wave loads one wave-uniform element per load through the scalar data
+ * cache, `unroll` in flight per wave (1..16); mode 2: both (1..8).  Grid:
+ * wg_per_cu workgroups of 256 threads per CU (1..32); about target_loads
+ * loads per launch.  out[0] = mean ms per launch over `reps` launches after
+ * one warm-up, out[1] = lane loads per launch, out[2] = wave-uniform loads
+ * per launch.  Allocates and frees its own buffers; synchronous. */
+int taco_hip_gather_ceiling(int32_t mode, int32_t unroll, int32_t log2_entries,
+                            int32_t wg_per_cu, int64_t target_loads, int32_t reps,
+                            double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,6 +135,8 @@ _SIGS = {
     "taco_hip_module_report": (c_int, [c_int, c_char_p]),
     "taco_hip_module_finish": (c_int, [c_int]),
     "taco_hip_host_read_bytes": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "taco_hip_gather_ceiling": (c_int, [c_int32, c_int32, c_int32, c_int32, c_int64, c_int32,
+                                        c_void_p]),
 }
 
 

@@ -290,6 +290,21 @@ def _recognise(stmt, kind, a):
             block = s2[4] * warp
             rows = s1[4] // block
             return (_lib.GEN_WARP_ROWS, block, 0, warp, max(rows, 1), 0)
+    if kind == "spmm" and ops == ["split", "split", "split", "parallelize", "parallelize",
+                                  "parallelize"]:
+        # scheduleSpMMRowsGPU: rows split over blocks and warps (R rows per
+        # warp), k split into 32-column chunks over the warp's threads.
+        s1, s2, s3 = sp
+        k = next(v for v in stmt.assignment.lhs.vars[1:])
+        ok = (s1[1] is i and s2[1] is s1[3] and s3[1] is k
+              and units.get(s1[2], (None,))[0] == ParallelUnit.GPUBlock
+              and units.get(s2[2], (None,))[0] == ParallelUnit.GPUWarp
+              and units.get(s3[2], (None, None)) == (ParallelUnit.GPUThread,
+                                                    OutputRaceStrategy.Atomics))
+        if ok and s1[4] % s2[4] == 0:
+            rows = s2[4]
+            block = (s1[4] // rows) * 32  # WARP_SIZE threads per warp
+            return (_lib.GEN_WARP_ROWS, block, 0, s3[4], rows, 0)
     if kind == "spmm" and ops == ["fuse", "pos", "split", "split", "split", "bound",
                                   "parallelize", "parallelize", "parallelize"]:
         fuse = next(c for c in cmds if c[0] == "fuse")

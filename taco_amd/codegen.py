@@ -10,6 +10,8 @@ from . import _lib
 # Named schedules: the reference's GPU schedules at their default parameters
 # (test/tests-scheduling-eval.cpp), the f32 forms, and the HIP backend's
 # merge-path SpMV and SpMM, and the block-per-row SpMV (GPUBlockReduction).
+# spmm_rows_f64 is scheduleSpMMRowsGPU (4 rows per warp, 32-column chunks);
+# spmm_merge16/32 are merge-path SpMM with teams of 16 / 32 lanes (K <= 128).
 STANDARD_SPECS = {
     "spmv_default_f64": (_lib.OP_SPMV, _lib.GEN_DEFAULT, 64, 256, 0, 0, 0, 0),
     "spmv_rows_f64": (_lib.OP_SPMV, _lib.GEN_WARP_ROWS, 64, 256, 0, 32, 1, 0),
@@ -23,6 +25,9 @@ STANDARD_SPECS = {
     "spmm_nnz_f64": (_lib.OP_SPMM, _lib.GEN_NNZ_SPLIT, 64, 256, 8, 32, 0, 4),
     "spmm_nnz_f32": (_lib.OP_SPMM, _lib.GEN_NNZ_SPLIT, 32, 256, 8, 32, 0, 4),
     "spmm_merge_f64": (_lib.OP_SPMM, _lib.GEN_MERGE, 64, 256, 256, 64, 0, 2),
+    "spmm_rows_f64": (_lib.OP_SPMM, _lib.GEN_WARP_ROWS, 64, 256, 0, 32, 4, 0),
+    "spmm_merge16_f64": (_lib.OP_SPMM, _lib.GEN_MERGE, 64, 256, 256, 16, 0, 8),
+    "spmm_merge32_f64": (_lib.OP_SPMM, _lib.GEN_MERGE, 64, 256, 256, 32, 0, 4),
     "spgemm_f64": (_lib.OP_SPGEMM, _lib.GEN_TWO_PHASE, 64, 256, 0, 0, 0, 0),
 }
 

@@ -386,6 +386,56 @@ Stmt spmm_default(const taco_hip_gen_spec& s, DType vt, const Csr& A,
       LoopKind::Serial, ParallelUnit::GPUBlock, OutputRaceStrategy::NoRaces);
 }
 
+// scheduleSpMMRowsGPU (tests-scheduling-eval.cpp:344-355): split(i, block,
+// i1, R*(B/32)), split(i1, warp, warp_row, R), split(k, thread, thread_col,
+// 32), reorder(block, warp, warp_row, thread, thread_col, j); block/warp
+// IgnoreRaces, thread Atomics.  Each warp owns R consecutive rows; its
+// threads own 32-column chunks of C's row (thread_col serial), each
+// summing the row's entries in order into C[i,k] with an atomic add (one
+// writer per element, so the order is the sequential one).  taco's thread
+// loop has extent ceil(K/32), known only at run time; here a lane group has
+// kRowsLanes lanes and lane t takes chunks t, t + kRowsLanes, ...  The
+// result is zero-filled first (initResultArrays).
+constexpr int kRowsLanes = 4;
+Stmt spmm_warp_rows(const taco_hip_gen_spec& s, DType vt, const Csr& A,
+                    const Dense& Bm, const Dense& C, std::vector<Stmt>& pre) {
+  const int B = s.block_size > 0 ? s.block_size : 256;
+  const int W = s.warp_size > 0 ? s.warp_size : 32;  // columns per chunk
+  const int R = s.rows_per_warp > 0 ? s.rows_per_warp : 4;
+  if (B % 32) throw std::runtime_error("WARP_ROWS SpMM: block size not a multiple of 32");
+  const int warps = B / 32, rows_per_tb = R * warps;
+  pre.push_back(launch("taco_hip_fill", {C.vals, mul(i64(C.dim1), i64(C.dim2)), zero(vt)}));
+  Expr block_v = var("block", DType::Int32), warp = var("warp", DType::Int32);
+  Expr warp_row = var("warp_row", DType::Int32), lane = var("thread_lane", DType::Int32);
+  Expr thread = var("thread", DType::Int32), tc = var("thread_col", DType::Int32);
+  Expr i = var("i", DType::Int32), k = var("k", DType::Int32);
+  Expr jA = var("jA", DType::Int32), j = var("j", DType::Int32);
+  Expr kC = var("kC", DType::Int64);
+  Stmt entries = for_loop(
+      jA, load(A.pos, i), load(A.pos, add(i, i32(1))), i32(1),
+      block({decl(j, load(A.crd, jA)),
+             store_add(C.vals, kC,
+                       mul(load(A.vals, jA), load(Bm.vals, add(mul(i64(j), i64(Bm.dim2)),
+                                                                 i64(k)))),
+                       true, ParallelUnit::GPUThread)}));
+  Stmt cols = for_loop(
+      tc, i32(0), i32(W), i32(1),
+      block({decl(k, add(mul(thread, i32(W)), tc)),
+             if_then(lt(k, Bm.dim2),
+                     block({decl(kC, add(mul(i64(i), i64(C.dim2)), i64(k))), entries}))}));
+  Stmt chunks = for_loop(thread, lane, ceil_div(Bm.dim2, W), i32(kRowsLanes), cols);
+  Stmt row = for_loop(
+      warp_row, i32(0), i32(R), i32(1),
+      block({decl(i, add(mul(block_v, i32(rows_per_tb)), add(mul(warp, i32(R)), warp_row))),
+             if_then(lt(i, A.dim1),
+                     for_loop(lane, i32(0), i32(kRowsLanes), i32(1), chunks, LoopKind::Serial,
+                              ParallelUnit::GPUThread, OutputRaceStrategy::Atomics))}));
+  return for_loop(block_v, i32(0), ceil_div(A.dim1, rows_per_tb), i32(1),
+                  for_loop(warp, i32(0), i32(warps), i32(1), row, LoopKind::Serial,
+                           ParallelUnit::GPUWarp),
+                  LoopKind::Serial, ParallelUnit::GPUBlock);
+}
+
 // scheduleSpMMGPU: reorder(i,j,k), fuse(i,j,f), pos(f), split(fpos, block,
 // fpos1, P*(B/W)), split(fpos1, warp, nnz, P), split(k, dense_val, thread,
 // W), reorder(block, warp, thread, dense_val, nnz), bound(dense_val, D);
@@ -594,6 +644,7 @@ std::vector<Stmt> lower_spmm(const taco_hip_gen_spec& s, DType vt) {
   Stmt k;
   switch (s.family) {
     case TACO_HIP_GEN_DEFAULT: k = spmm_default(s, vt, A, Bm, C, pre); break;
+    case TACO_HIP_GEN_WARP_ROWS: k = spmm_warp_rows(s, vt, A, Bm, C, pre); break;
     case TACO_HIP_GEN_NNZ_SPLIT: k = spmm_nnz_split(s, vt, A, Bm, C, pre); break;
     case TACO_HIP_GEN_MERGE: k = spmm_merge(s, vt, A, Bm, C, pre); break;
     default: throw std::runtime_error("SpMM: unsupported schedule family");

@@ -79,6 +79,21 @@ def schedule_spmm_gpu(stmt, A, precomputed, i, j, k, NNZ_PER_WARP=8, BLOCK_SIZE=
             .parallelize(thread, PU.GPUThread, ORS.Atomics))
 
 
+def schedule_spmm_rows_gpu(stmt, A, precomputed, i, j, k, ROWS_PER_WARP=4, BLOCK_SIZE=256):
+    """scheduleSpMMRowsGPU (tests-scheduling-eval.cpp:344-355): the same
+    number of rows per warp, each thread of a warp a 32-column chunk of k."""
+    ROWS_PER_TB = ROWS_PER_WARP * (BLOCK_SIZE // WARP_SIZE)
+    i1, block, warp = IndexVar("i1"), IndexVar("block"), IndexVar("warp")
+    warp_row, thread, thread_col = IndexVar("warp_row"), IndexVar("thread"), IndexVar("thread_col")
+    return (stmt.split(i, block, i1, ROWS_PER_TB)
+            .split(i1, warp, warp_row, ROWS_PER_WARP)
+            .split(k, thread, thread_col, 32)
+            .reorder([block, warp, warp_row, thread, thread_col, j])
+            .parallelize(block, PU.GPUBlock, ORS.IgnoreRaces)
+            .parallelize(warp, PU.GPUWarp, ORS.IgnoreRaces)
+            .parallelize(thread, PU.GPUThread, ORS.Atomics))
+
+
 def schedule_spmm_merge_gpu(stmt, A, precomputed, i, j, k, ITEMS_PER_TEAM=256, BLOCK_SIZE=256,
                             LANES=64, COLS_PER_LANE=2):
     """The HIP backend's merge-path SpMM: scheduleSpMMGPU's loop structure

@@ -60,6 +60,10 @@ def test_reference_spmm_schedule():
     assert recognise(C, st) == ("spmm", (_lib.GEN_NNZ_SPLIT, 256, 8, 32, 0, 4))
     st = S.schedule_spmm_merge_gpu(C.get_assignment().concretize(), A, pre, i, j, k)
     assert recognise(C, st) == ("spmm", (_lib.GEN_MERGE, 256, 256, 64, 0, 2))
+    st = S.schedule_spmm_rows_gpu(C.get_assignment().concretize(), A, pre, i, j, k)
+    assert recognise(C, st) == ("spmm", (_lib.GEN_WARP_ROWS, 256, 0, 32, 4, 0))
+    st = S.schedule_spmm_rows_gpu(C.get_assignment().concretize(), A, pre, i, j, k, 2, 128)
+    assert recognise(C, st) == ("spmm", (_lib.GEN_WARP_ROWS, 128, 0, 32, 2, 0))
     st = S.schedule_spmm_merge_gpu(C.get_assignment().concretize(), A, pre, i, j, k, 64, 128,
                                    32, 4)
     assert recognise(C, st) == ("spmm", (_lib.GEN_MERGE, 128, 64, 32, 0, 4))

@@ -42,6 +42,18 @@ def test_ir_warp_rows_units():
     assert "threadIdx.x / 32" in src and "threadIdx.x % 32" in src
 
 
+def test_ir_spmm_warp_rows():
+    """scheduleSpMMRowsGPU: 8 warps of R = 4 rows per 256-thread block, k in
+    32-column chunks over the warp's lanes, atomic adds into C."""
+    ir = _lib.generate(codegen.standard("spmm_rows_f64"), 1)
+    for unit in ("GPUBlock", "GPUWarp", "GPUThread"):
+        assert unit in ir
+    src = _lib.generate(codegen.standard("spmm_rows_f64"))
+    assert "((block * 32) + ((warp * 4) + warp_row))" in src
+    assert "((B2_dimension + 31) / 32)" in src
+    assert "taco_hip_atomicAdd(C_vals" in src and "taco_hip_fill(C_vals" in src
+
+
 def test_ir_nnz_split_atomics():
     ir = _lib.generate(codegen.standard("spmv_nnz_f64"), 1)
     assert "[atomic GPUThread]" in ir and "[atomic GPUWarp]" in ir

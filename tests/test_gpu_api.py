@@ -65,7 +65,7 @@ def test_spmv_gpu_reference_inputs(schedule):
     assert taco.equals(y, expected)  # taco's own predicate, as its GPU tests use it
 
 
-@pytest.mark.parametrize("schedule", ["spmm_gpu", "merge"])
+@pytest.mark.parametrize("schedule", ["spmm_gpu", "merge", "rows"])
 def test_spmm_gpu_reference_inputs(schedule):
     d = np.load(os.path.join(GOLD, "ref_tests.npz"))
     i, j, k = taco.IndexVar("i"), taco.IndexVar("j"), taco.IndexVar("k")
@@ -75,7 +75,8 @@ def test_spmm_gpu_reference_inputs(schedule):
     C = taco.Tensor("C", [102, 128], taco.dense_matrix)
     precomputed = A[i, j] * B[j, k]
     C[i, k] = precomputed
-    sched = S.schedule_spmm_gpu if schedule == "spmm_gpu" else S.schedule_spmm_merge_gpu
+    sched = {"spmm_gpu": S.schedule_spmm_gpu, "merge": S.schedule_spmm_merge_gpu,
+             "rows": S.schedule_spmm_rows_gpu}[schedule]
     stmt = sched(C.get_assignment().concretize(), A, precomputed, i, j, k)
     C.compile(stmt)
     C.assemble()

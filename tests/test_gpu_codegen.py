@@ -19,7 +19,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 SPMV = ["spmv_default_f64", "spmv_rows_f64", "spmv_rows64_f64", "spmv_nnz_f64", "spmv_merge_f64",
         "spmv_block_f64"]
-SPMM = ["spmm_default_f64", "spmm_nnz_f64", "spmm_merge_f64"]
+SPMM = ["spmm_default_f64", "spmm_nnz_f64", "spmm_merge_f64", "spmm_rows_f64"]
 
 
 def mats():
@@ -194,6 +194,38 @@ def test_generated_spgemm():
     assert oracle.error_report(vals, ex, cond, 1e-12)["ok"]
     hp, hc, hv = G.spgemm(A, B)
     assert np.array_equal(hp, pos) and np.array_equal(hc, crd) and np.array_equal(hv, vals)
+
+
+def test_generated_spgemm_rows_strided_and_hashed():
+    """More rows than the compute grid holds at once (4 workgroups of 4
+    waves per CU), so rows are strided over the resident grid, and rows with
+    more than 64 candidates, so one wave reuses its LDS hash accumulator
+    across consecutive long rows."""
+    A = W.csr_powerlaw(20000, 15000, seed=37, kmax=100)
+    B = W.csr_fixed(15000, 12000, 8, 38)
+    assert (np.diff(A.pos) * 8 > 64).sum() > 100
+    pos, crd, vals = run_spgemm(A, B)
+    opos, ocrd = oracle.spgemm_assemble(A, B)
+    assert np.array_equal(pos, opos) and np.array_equal(crd, ocrd)
+    ex, cond = oracle.exact_spgemm_values(A, B, opos, ocrd)
+    assert oracle.error_report(vals, ex, cond, 1e-12)["ok"]
+    hp, hc, hv = G.spgemm(A, B)
+    assert np.array_equal(hp, pos) and np.array_equal(hc, crd) and np.array_equal(hv, vals)
+
+
+@pytest.mark.parametrize("name", ["spmm_merge16_f64", "spmm_merge32_f64"])
+@pytest.mark.parametrize("k", [20, 128])
+def test_generated_merge_spmm_narrow_teams(name, k):
+    """Merge-path SpMM with lane teams narrower than a wave: several teams
+    share one wave and broadcast their entries through LDS permutes."""
+    A = W.csr_powerlaw(5000, 4000, seed=39, kmax=3000)
+    B = W.uniform(A.n * k, 39, W.STREAM_B).reshape(A.n, k)
+    C = torch.full((A.m, k), float("nan"), dtype=torch.float64, device="cuda")
+    kernel(name).compute(D.dense(C), G.csr_desc(A), D.dense(G.dev(B)))
+    torch.cuda.synchronize()
+    Ce, cond = oracle.exact_spmm(A, B)
+    rep = oracle.error_report(C.cpu().numpy(), Ce, cond, 1e-12)
+    assert rep["ok"], rep
 
 
 def test_generated_spgemm_known_answer():

@@ -41,29 +41,68 @@ def test_powerlaw_spmv_merge_path(powerlaw):
     assert oracle.error_report(y, ye, cond, 1e-12)["ok"]
 
 
-def test_powerlaw_spmm_f32_k64_first_rows():
-    """SpMM on the full matrix; checked on a row sample (the exact oracle of
-    all 128M outputs would take minutes on the CPU)."""
+@pytest.fixture(scope="module")
+def powerlaw_spmm():
+    """The 2M power-law A with a dense 2M x 64 B, and the row sample the
+    SpMM checks use: every 9973rd row plus the 50 longest rows."""
     A = W.powerlaw_2m()
     K = 64
     B = W.uniform(A.n * K, 2, W.STREAM_B).reshape(A.n, K)
-    a = G.csr_desc(A, "f32")
-    Bd = G.dev(B, torch.float32)
-    C = torch.full((A.m, K), float("nan"), dtype=torch.float32, device="cuda")
-    s = _lib.Schedule()
-    L = _lib.lib()
-    L.taco_hip_default_schedule(1, A.m, A.nnz, K, s)
-    _lib.check(L.taco_hip_spmm_compute(D.dense(C, "f32").ptr, a.ptr, D.dense(Bd, "f32").ptr, s))
-    torch.cuda.synchronize()
-    rows = np.unique(np.concatenate([np.arange(0, A.m, 9973), np.argsort(np.diff(A.pos))[-50:]]))
+    rows = np.unique(np.concatenate([np.arange(0, A.m, 9973), np.argsort(np.diff(A.pos))[-50:],
+                                     [A.m - 1]]))
+    return A, B, rows
+
+
+def sample_rows(A, rows, vals):
     sub_pos = np.concatenate([[0], np.cumsum(np.diff(A.pos)[rows])]).astype(np.int32)
     sub_crd = np.concatenate([A.crd[A.pos[r]:A.pos[r + 1]] for r in rows])
-    sub_val = np.concatenate([A.vals[A.pos[r]:A.pos[r + 1]] for r in rows])
-    As = W.Csr(len(rows), A.n, sub_pos, sub_crd, sub_val.astype(np.float32).astype(np.float64))
-    Ce, cond = oracle.exact_spmm(As, B.astype(np.float32).astype(np.float64))
-    got = C[torch.from_numpy(rows).cuda()].cpu().numpy()
-    rep = oracle.error_report(got, Ce, cond, 1e-5)
+    sub_val = np.concatenate([vals[A.pos[r]:A.pos[r + 1]] for r in rows])
+    return W.Csr(len(rows), A.n, sub_pos, sub_crd, sub_val)
+
+
+# (dtype, schedule, tolerance against the FP64 oracle of the unrounded
+# inputs); bf16 is also held to the f32 tolerance on its rounded inputs.
+FULL_SPMM = [("f64", "default", 1e-12), ("f32", "default", 1e-5), ("bf16", "mfma", 1e-2)]
+
+
+@pytest.mark.parametrize("dtype,which,tol", FULL_SPMM, ids=[f[0] for f in FULL_SPMM])
+def test_powerlaw_spmm_k64_row_sample(powerlaw_spmm, dtype, which, tol):
+    """SpMM on the full 2M-row matrix with K = 64, in each precision form the
+    benchmark times; checked on a row sample (the exact oracle of all 128M
+    outputs would take minutes on the CPU)."""
+    A, B, rows = powerlaw_spmm
+    K = B.shape[1]
+    print(f"full-size SpMM {dtype} ({which}), nnz={A.nnz}", flush=True)
+    L = _lib.lib()
+    s = _lib.Schedule()
+    if which == "mfma":
+        s = G.sched(_lib.MFMA)
+    else:
+        L.taco_hip_default_schedule(1, A.m, A.nnz, K, s)
+    a = G.csr_desc(A, dtype)
+    if dtype == "bf16":
+        bdesc_t = G.bf16_bits(B).to("cuda")
+        C = torch.full((A.m, K), float("nan"), dtype=torch.float32, device="cuda")
+        bdesc, cdesc = D.dense(bdesc_t, "bf16"), D.dense(C, "f32")
+    else:
+        tdt = torch.float64 if dtype == "f64" else torch.float32
+        bdesc_t = G.dev(B, tdt)
+        C = torch.full((A.m, K), float("nan"), dtype=tdt, device="cuda")
+        bdesc, cdesc = D.dense(bdesc_t, dtype), D.dense(C, dtype)
+    _lib.check(L.taco_hip_spmm_compute(cdesc.ptr, a.ptr, bdesc.ptr, s))
+    torch.cuda.synchronize()
+    got = C[torch.from_numpy(rows).cuda()].cpu().numpy().astype(np.float64)
+    del C, bdesc_t, a
+    # the oracle of the inputs as given ...
+    Ce, cond = oracle.exact_spmm(sample_rows(A, rows, A.vals), B)
+    rep = oracle.error_report(got, Ce, cond, tol)
     assert rep["ok"], rep
+    # ... and, for the reduced precisions, of the inputs as the GPU saw them
+    if dtype != "f64":
+        rnd = G.bf16_values if dtype == "bf16" else (lambda v: v.astype(np.float32).astype(np.float64))
+        Cr, condr = oracle.exact_spmm(sample_rows(A, rows, rnd(A.vals)), rnd(B))
+        rep = oracle.error_report(got, Cr, condr, 1e-5)
+        assert rep["ok"], rep
 
 
 def test_spgemm_500k_pattern_bit_exact():
