@@ -149,6 +149,22 @@ void taco_hip_default_schedule(int32_t op, int32_t m, int64_t nnz, int32_t k,
 int taco_hip_spmv_compute(taco_tensor_t* y, taco_tensor_t* A,
                           taco_tensor_t* x, const taco_hip_schedule* sched);
 
+/* No counterpart in the reference: taco_hip_spmv_compute for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, schedules and
+ * results; it calls no hipMalloc/hipFree.  The MERGE_PATH schedule's
+ * carries go to a static 1 MiB device array (one per GPU, held until the
+ * process ends) instead of the library workspace, which bounds that
+ * schedule to (rows + entries) / (256 * items_per_thread) workgroups of
+ * 12 bytes (f64) or 8 bytes (f32) each: 87,376 workgroups in f64, i.e.
+ * 178.9M rows + entries at 8 items per thread; larger problems are rejected
+ * with an error before anything is launched.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (several streams or host
+ * threads) race.  Nothing in it needs clearing between calls (every
+ * workgroup writes its carry before the fix-up reads it). */
+int taco_hip_spmv_compute_noalloc(taco_tensor_t* y, taco_tensor_t* A,
+                                  taco_tensor_t* x, const taco_hip_schedule* sched);
+
 /* C(i,k) = A(i,j) * B(j,k).  A CSR; B, C dense row-major.  f64, f32, and
  * bf16 A/B with f32 C (A->csize == B->csize == 16, C->csize == 32). */
 int taco_hip_spmm_compute(taco_tensor_t* C, taco_tensor_t* A,
@@ -372,6 +388,35 @@ void taco_hip_module_destroy(taco_hip_module* m);
 int taco_hip_module_report(int code, const char* what);
 int taco_hip_module_finish(int rc);
 int taco_hip_host_read_bytes(void* dst, const void* src, size_t bytes);
+
+/* ======================================================================
+ * Tensor files (reference include/taco/storage/file_io_mtx.h, file_io_tns.h;
+ * src/storage/file_io_mtx.cpp:37-160, file_io_tns.cpp:30-87): MatrixMarket
+ * (.mtx, .ttx) and FROSTT (.tns) read into a coordinate list, which the
+ * caller packs (taco's read() inserts every coordinate and packs).
+ * ==================================================================== */
+
+typedef struct taco_hip_coo taco_hip_coo;
+/* Reads `path` by its extension.  MatrixMarket: "real" field, "general" or
+ * "symmetric" (the mirrored entry is added for i != j), "coordinate" or
+ * "array" (every value, first mode fastest); other headers are errors, as
+ * in taco.  FROSTT: the order is the first line's token count minus one,
+ * each dimension the largest coordinate in its mode. */
+int taco_hip_read_tensor_file(const char* path, taco_hip_coo** out);
+/* order, dims[0..min(order, max_dims)), entry count, and whether the file
+ * was a dense array. */
+int taco_hip_coo_info(const taco_hip_coo* coo, int32_t* order, int32_t* dims,
+                      int32_t max_dims, int64_t* nnz, int32_t* dense);
+/* Copies the 0-based coordinates (nnz x order, row-major) and values. */
+int taco_hip_coo_data(const taco_hip_coo* coo, int32_t* coords, double* vals);
+void taco_hip_coo_destroy(taco_hip_coo* coo);
+/* writeMTX / writeTNS (file_io_mtx.cpp:233-290, file_io_tns.cpp:89-130):
+ * by extension; dense = 1 writes a MatrixMarket array (vals holds every
+ * position, first mode fastest; coords unused).  Values with 17 significant
+ * digits. */
+int taco_hip_write_tensor_file(const char* path, int32_t order, const int32_t* dims,
+                               int64_t nnz, const int32_t* coords, const double* vals,
+                               int32_t dense);
 
 /* ======================================================================
  * Measurement.  No reference counterpart: a bound for the gather-limited

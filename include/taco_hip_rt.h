@@ -617,6 +617,120 @@ __device__ inline int32_t taco_hip_spgemm_row_symbolic(
   return cnt;
 }
 
+/* Numeric row of more than TACO_HIP_ROW_CAP columns: C's values are the
+ * accumulator, zeroed first.  Candidates are taken 64 at a time in
+ * encounter order (jA outer, kB inner), across A's entries: lane t first
+ * reads entry p+t's B row bounds, a wave prefix sum of the row lengths
+ * places candidate q in lane q.  Each candidate finds its slot by binary
+ * search in the row's sorted pattern; the chunk is sorted by (slot, lane),
+ * and the first lane of each slot's run adds the run's products to the slot
+ * one after another.  Every column so sums its products in encounter
+ * order, as taco's workspace loop does. */
+__device__ inline void taco_hip_spgemm_row_numeric_global(
+    const int32_t* apos, const int32_t* acrd, const double* avals, const int32_t* bpos,
+    const int32_t* bcrd, const double* bvals, int32_t i, int32_t c0, int32_t nc,
+    const int32_t* ccrd, double* cvals) {
+  const int lane = (int)__lane_id();
+  const int32_t* crow = ccrd + c0;
+  double* out = cvals + c0;
+  for (int32_t q = lane; q < nc; q += 64) out[q] = 0.0;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  const int32_t a1 = apos[i + 1];
+  int32_t p = apos[i], off = 0; /* next candidate: entry p, offset off in its B row */
+  while (p < a1) {
+    int32_t b0 = 0, avail = 0;
+    double av = 0.0;
+    if (p + lane < a1) {
+      const int32_t j = acrd[p + lane];
+      b0 = bpos[j];
+      avail = bpos[j + 1] - b0;
+      av = avals[p + lane];
+      if (lane == 0) {
+        b0 += off;
+        avail -= off;
+      }
+    }
+    /* lengths clamped to 65: sums stay in int32, a total above 64 still
+     * reads as above 64 */
+    const int32_t lc = avail < 65 ? avail : 65;
+    int32_t inc = lc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t u = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += u;
+    }
+    const int32_t tot = __shfl(inc, 63, 64);
+    const int32_t used = tot < 64 ? tot : 64;
+    /* entry of candidate `lane`: the number of entries whose inclusive sum
+     * is <= lane */
+    int t = 0;
+    for (int s2 = 0; s2 < 63; ++s2) t += __shfl(inc, s2, 64) <= lane ? 1 : 0;
+    const int tq = t < 64 ? t : 63;
+    const int32_t excl = __shfl(inc - lc, tq, 64);
+    const int32_t bt = __shfl(b0, tq, 64);
+    const double at = __shfl(av, tq, 64);
+    const bool valid = lane < used;
+    uint32_t slot = 0xFFFFFFFFu;
+    double prod = 0.0;
+    if (valid) {
+      const int32_t kB = bt + (lane - excl);
+      const int32_t col = bcrd[kB];
+      prod = at * bvals[kB];
+      int32_t lo = 0, hi = nc;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (crow[mid] < col) lo = mid + 1; else hi = mid;
+      }
+      if (lo < nc && crow[lo] == col) slot = (uint32_t)lo;
+    }
+    uint64_t key = ((uint64_t)slot << 32) | (uint32_t)lane;
+    key = taco_hip_wave_sort(key);
+    const uint32_t sslot = (uint32_t)(key >> 32);
+    const double v = __shfl(prod, (int)(key & 63), 64);
+    const uint32_t prev = __shfl_up(sslot, 1, 64);
+    const bool live = sslot != 0xFFFFFFFFu;
+    const bool head = live && (lane == 0 || prev != sslot);
+    const uint64_t heads = __ballot(head);
+    const int nlive = __popcll(__ballot(live));
+    int run = 0;
+    if (head) {
+      const uint64_t above = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1);
+      const int next = above ? __ffsll((unsigned long long)above) - 1 : nlive;
+      run = next - lane;
+    }
+    int maxrun = run;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const int x = __shfl_xor(maxrun, o, 64);
+      maxrun = x > maxrun ? x : maxrun;
+    }
+    double acc = 0.0;
+    if (head) acc = out[sslot] + v;
+    for (int q = 1; q < maxrun; ++q) {
+      const double u = __shfl(v, (lane + q) & 63, 64);
+      if (head && q < run) acc = acc + u;
+    }
+    if (head) out[sslot] = acc;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    /* advance past the `used` candidates */
+    if (used == 0) {
+      p += 64;
+      off = 0;
+    } else {
+      const int tl = __shfl(t, used - 1, 64);
+      const int32_t done = used - __shfl(inc - lc, tl, 64); /* taken from entry tl */
+      const int32_t left = __shfl(avail, tl, 64) - done;
+      if (left == 0) {
+        off = 0;
+        p += tl + 1;
+      } else {
+        off = (tl == 0 ? off : 0) + done;
+        p += tl;
+      }
+    }
+  }
+}
+
 /* Numeric row on an assembled (sorted) pattern. */
 __device__ inline void taco_hip_spgemm_row_numeric(
     const int32_t* apos, const int32_t* acrd, const double* avals,
@@ -668,7 +782,11 @@ __device__ inline void taco_hip_spgemm_row_numeric(
     }
     return;
   }
-  if (nc > TACO_HIP_ROW_CAP) return; /* pattern not from the symbolic pass */
+  if (nc > TACO_HIP_ROW_CAP) {
+    taco_hip_spgemm_row_numeric_global(apos, acrd, avals, bpos, bcrd, bvals, i, c0, nc, ccrd,
+                                       cvals);
+    return;
+  }
   double* acc = s_acc[(threadIdx.x / 64) % TACO_HIP_ROW_WAVES];
   for (int q = lane; q < nc; q += 64) acc[q] = 0;
   taco_hip_wave_lds_sync();

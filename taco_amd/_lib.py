@@ -81,6 +81,7 @@ _SIGS = {
     "taco_hip_default_schedule": (None, [c_int32, c_int32, c_int64, c_int32,
                                          POINTER(Schedule)]),
     "taco_hip_spmv_compute": (c_int, [P_T, P_T, P_T, POINTER(Schedule)]),
+    "taco_hip_spmv_compute_noalloc": (c_int, [P_T, P_T, P_T, POINTER(Schedule)]),
     "taco_hip_spmm_compute": (c_int, [P_T, P_T, P_T, POINTER(Schedule)]),
     "taco_hip_spgemm_assemble": (c_int, [P_T, P_T, P_T]),
     "taco_hip_spgemm_compute": (c_int, [P_T, P_T, P_T]),
@@ -135,6 +136,13 @@ _SIGS = {
     "taco_hip_module_report": (c_int, [c_int, c_char_p]),
     "taco_hip_module_finish": (c_int, [c_int]),
     "taco_hip_host_read_bytes": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "taco_hip_read_tensor_file": (c_int, [c_char_p, POINTER(c_void_p)]),
+    "taco_hip_coo_info": (c_int, [c_void_p, POINTER(c_int32), c_void_p, c_int32,
+                                  POINTER(c_int64), POINTER(c_int32)]),
+    "taco_hip_coo_data": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "taco_hip_coo_destroy": (None, [c_void_p]),
+    "taco_hip_write_tensor_file": (c_int, [c_char_p, c_int32, c_void_p, c_int64, c_void_p,
+                                           c_void_p, c_int32]),
     "taco_hip_gather_ceiling": (c_int, [c_int32, c_int32, c_int32, c_int32, c_int64, c_int32,
                                         c_void_p]),
 }

@@ -343,6 +343,81 @@ def _recognise(stmt, kind, a):
     raise TacoError(f"schedule not supported by the HIP backend: {ops}")
 
 
+# ----------------------------------------------------- learned schedules --
+
+_learned = {"enabled": False, "selector": None}
+
+
+def set_learned_schedule_enabled(enabled):
+    """With it on, compile() of an SpMV or SpMM statement that carries no
+    scheduling commands — where taco applies its default GPU schedule — lets
+    the learned selector (taco_amd.selector) pick the schedule at the first
+    compute() on each operand matrix: one of the hand-written schedules, or
+    for f64 SpMV the slab plan, which is built once per matrix, kept on the
+    operand tensor and dropped when that tensor's values change.  The choice
+    changes which correct kernel runs, never the result beyond the value
+    tolerance.  Off by default: taco's default schedule is the default."""
+    _learned["enabled"] = bool(enabled)
+
+
+def is_learned_schedule_enabled():
+    return _learned["enabled"]
+
+
+class _Learned:
+    """The compute() of a statement scheduled by the learned selector."""
+
+    def __init__(self, kind, A):
+        self.op = 0 if kind == "spmv" else 1
+        self.A = A
+
+    def _choose(self, a_desc, K):
+        from . import autotune, selector
+        if _learned["selector"] is None:
+            _learned["selector"] = selector.Selector()
+        sel = _learned["selector"]
+        z = sel.scores(selector.features(a_desc, self.op, K))[0]
+        lo, hi = selector.class_range(self.op)
+        for c in lo + np.argsort(-z[lo:hi], kind="stable"):
+            s = selector.class_schedule(int(c), K)
+            if s is None:  # the slab plan: f64 only, within its limits
+                plan, _ = autotune.slab_plan(a_desc)
+                if plan is None:
+                    continue
+                self.A._plan = plan
+            return int(c), s
+        raise TacoError("learned schedule: no applicable class")
+
+    def assemble(self, *descs):
+        pass
+
+    def compute(self, out_desc, a_desc, b_desc):
+        K = 0 if self.op == 0 else int(out_desc.t.dimensions[1])
+        choice = self.A._schedule_choice.get((self.op, K))
+        if choice is None:
+            choice = self._choose(a_desc, K)
+            self.A._schedule_choice[(self.op, K)] = choice
+        c, s = choice
+        L = _lib.lib()
+        if s is None:
+            if self.A._plan is None:
+                from . import autotune
+                self.A._plan, _ = autotune.slab_plan(a_desc)
+            self.A._plan.compute(out_desc, b_desc)
+        elif self.op == 0:
+            _lib.check(L.taco_hip_spmv_compute(out_desc.ptr, a_desc.ptr, b_desc.ptr, s),
+                       "learned SpMV")
+        else:
+            _lib.check(L.taco_hip_spmm_compute(out_desc.ptr, a_desc.ptr, b_desc.ptr, s),
+                       "learned SpMM")
+
+    def chosen(self, K=0):
+        """Name of the class picked for the operand (None before compute)."""
+        from . import selector
+        ch = self.A._schedule_choice.get((self.op, K))
+        return None if ch is None else selector.CLASS_NAMES[ch[0]]
+
+
 # --------------------------------------------------------------- tensors --
 
 class Tensor:
@@ -364,13 +439,25 @@ class Tensor:
         self._desc = None
         self._dev = None
         self._state = "new"
+        self._plan = None            # slab plan of this matrix (learned schedules)
+        self._schedule_choice = {}   # (op, K) -> (class, schedule) picked for it
+
+    def _values_changed(self):
+        """Drops what was derived from the values: the slab plan and the
+        learned schedule choices."""
+        self._plan = None
+        self._schedule_choice = {}
 
     # -- values
     def insert(self, coords, value):
         if len(coords) != len(self.dims):
             raise TacoError(f"{self.name}: {len(coords)} coordinates for order {len(self.dims)}")
-        self._coo[tuple(int(c) for c in coords)] = value
+        # taco's insert appends to a coordinate buffer; pack sums the values
+        # of repeated coordinates (test/tests-tensor.cpp:68-79)
+        key = tuple(int(c) for c in coords)
+        self._coo[key] = self._coo.get(key, 0.0) + value
         self._host = None
+        self._values_changed()
 
     def pack(self):
         if self.format == CSR:
@@ -389,6 +476,7 @@ class Tensor:
                 d[k] = v
             self._host = d
         self._dev = None
+        self._values_changed()
 
     def set_csr(self, pos, crd, vals):
         """Host CSR arrays (taco's pack() result) for a CSR tensor."""
@@ -397,6 +485,7 @@ class Tensor:
         self._host = (np.ascontiguousarray(pos, np.int32), np.ascontiguousarray(crd, np.int32),
                       np.ascontiguousarray(vals, self.dtype.np))
         self._dev = None
+        self._values_changed()
 
     def set_values(self, array):
         """Dense values for a dense tensor."""
@@ -405,6 +494,7 @@ class Tensor:
             raise TacoError(f"{self.name}: values of shape {a.shape}, dims {self.dims}")
         self._host = a
         self._dev = None
+        self._values_changed()
 
     def _ensure_host(self):
         if self._host is None:
@@ -446,6 +536,14 @@ class Tensor:
         if stmt.assignment.lhs.tensor is not self:
             raise TacoError(f"statement {stmt!r} does not assign {self.name}")
         kind, a, b = _statement_kind(stmt.assignment)
+        if _learned["enabled"] and not stmt.commands and kind in ("spmv", "spmm"):
+            if a.tensor.dtype is not self.dtype or b.tensor.dtype is not self.dtype:
+                raise TacoError("operands and result must share one datatype")
+            self._kernel = _Learned(kind, a.tensor)
+            self._operands = (a.tensor, b.tensor)
+            self._kind = kind
+            self._state = "compiled"
+            return
         family, block, per, warp, rows, bound = _recognise(stmt, kind, a)
         op = {"spmv": _lib.OP_SPMV, "spmm": _lib.OP_SPMM, "spgemm": _lib.OP_SPGEMM}[kind]
         if a.tensor.dtype is not self.dtype or b.tensor.dtype is not self.dtype:
@@ -500,6 +598,7 @@ class Tensor:
         self._kernel.compute(self._desc, a._descriptor(), b._descriptor())
         torch.cuda.synchronize()
         self._host = None
+        self._values_changed()
         self._state = "computed"
 
     def evaluate(self):
@@ -595,3 +694,18 @@ def equals(a, b):
         ia += 1
         ib += 1
     return bool(np.all(av[ia:] == 0) and np.all(bv[ib:] == 0))
+
+
+# ----------------------------------------------------------------- files --
+
+def read(filename, fmt=None, name=None):
+    """taco::read (reference include/taco/tensor.h read(), file_io_*.cpp):
+    a packed Tensor from a MatrixMarket (.mtx, .ttx) or FROSTT (.tns) file."""
+    from . import io
+    return io.read(filename, fmt, name)
+
+
+def write(filename, tensor):
+    """taco::write: MatrixMarket or FROSTT by the file's extension."""
+    from . import io
+    io.write(filename, tensor)

@@ -19,7 +19,11 @@ Two forms:
     is ranked last with time inf.
 
 The candidate spaces are the schedule families the kernels implement; they
-are also the classes the learned selector predicts.
+are also the classes the learned selector predicts.  The SpMV space includes
+the slab plan (taco_hip_spmv_plan_*, f64): a one-time reordering of A that is
+then reused by every call on the same matrix.  Its build time is charged to
+the calls that reuse it, PLAN_CALLS of them: its candidate time is the
+per-call time plus build time / PLAN_CALLS.
 """
 import numpy as np
 
@@ -34,7 +38,12 @@ def _sched(algo, block=0, items=0, group=0, rows=0):
     return s
 
 
-# The SpMV schedule space (also the selector's SpMV classes).
+# Calls that share one slab plan's build cost (an iterative solver's
+# products with one matrix).
+PLAN_CALLS = 100
+
+# The SpMV schedule space (also the selector's SpMV classes); None is the
+# slab plan.
 SPMV_CANDIDATES = [
     ("thread_per_row", _sched(_lib.THREAD_PER_ROW, 256)),
     ("group2", _sched(_lib.GROUP_PER_ROW, 256, group=2)),
@@ -45,7 +54,7 @@ SPMV_CANDIDATES = [
     ("group64", _sched(_lib.GROUP_PER_ROW, 256, group=64)),
     ("nnz_split4", _sched(_lib.NNZ_SPLIT, 256, items=4)),
     ("nnz_split8", _sched(_lib.NNZ_SPLIT, 256, items=8)),
-    ("nnz_split16", _sched(_lib.NNZ_SPLIT, 256, items=16)),
+    ("slab_plan", None),
     ("merge4", _sched(_lib.MERGE_PATH, 256, items=4)),
     ("merge8", _sched(_lib.MERGE_PATH, 256, items=8)),
     ("merge16", _sched(_lib.MERGE_PATH, 256, items=16)),
@@ -91,9 +100,14 @@ def accepts(out, ref, rtol):
 
 
 def _tune(runs, result, reps, rtol):
+    """runs: (name, fn, extra_ms) with fn None for a candidate that cannot
+    run; extra_ms is added to its per-call time."""
     import torch
     out, ref = [], None
-    for name, fn in runs:
+    for name, fn, extra in runs:
+        if fn is None:
+            out.append((name, float("inf")))
+            continue
         fn()
         torch.cuda.synchronize()
         if ref is None:
@@ -101,17 +115,45 @@ def _tune(runs, result, reps, rtol):
         elif not accepts(result, ref, rtol):
             out.append((name, float("inf")))
             continue
-        out.append((name, _time(fn, reps)))
+        out.append((name, _time(fn, reps) + extra))
     return sorted(out, key=lambda t: t[1])
+
+
+def slab_plan(a_desc):
+    """(plan, build ms), or (None, inf) when the plan does not apply (f32
+    values, or a shape outside its limits)."""
+    import time
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    try:
+        plan = _lib.SpmvPlan(a_desc)
+    except _lib.TacoHipError:
+        return None, float("inf")
+    torch.cuda.synchronize()
+    return plan, (time.perf_counter() - t0) * 1e3
+
+
+def spmv_runner(s, a_desc, x_desc, y_desc, plan=None):
+    """The call that runs SpMV schedule s (None: the slab plan `plan`)."""
+    L = _lib.lib()
+    if s is None:
+        if plan is None:
+            return None
+        return lambda: plan.compute(y_desc, x_desc)
+    return lambda: _lib.check(L.taco_hip_spmv_compute(y_desc.ptr, a_desc.ptr, x_desc.ptr, s))
 
 
 def tune_spmv(a_desc, x_desc, y_desc, reps=5, candidates=None, rtol=1e-10):
     """Times every SpMV schedule on one matrix; returns [(name, ms)] sorted
-    (inf: output rejected).  y_desc must describe a device tensor."""
-    L = _lib.lib()
-    runs = [(name, lambda s=s: _lib.check(
-        L.taco_hip_spmv_compute(y_desc.ptr, a_desc.ptr, x_desc.ptr, s)))
-        for name, s in candidates or SPMV_CANDIDATES]
+    (inf: output rejected, or the plan does not apply).  y_desc must describe
+    a device tensor.  The slab plan's time includes its build time /
+    PLAN_CALLS."""
+    runs = []
+    for name, s in candidates or SPMV_CANDIDATES:
+        plan, extra = (slab_plan(a_desc) if s is None else (None, 0.0))
+        runs.append((name, spmv_runner(s, a_desc, x_desc, y_desc, plan), extra / PLAN_CALLS
+                     if s is None else 0.0))
     return _tune(runs, y_desc.keep[0], reps, rtol)
 
 
@@ -120,7 +162,7 @@ def tune_spmm(a_desc, b_desc, c_desc, K, reps=3, candidates=None, rtol=1e-4):
     (inf: output rejected).  c_desc must describe a device tensor."""
     L = _lib.lib()
     runs = [(name, lambda s=s: _lib.check(
-        L.taco_hip_spmm_compute(c_desc.ptr, a_desc.ptr, b_desc.ptr, s)))
+        L.taco_hip_spmm_compute(c_desc.ptr, a_desc.ptr, b_desc.ptr, s)), 0.0)
         for name, s in candidates or spmm_candidates(K)]
     return _tune(runs, c_desc.keep[0], reps, rtol)
 

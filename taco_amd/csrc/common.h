@@ -32,6 +32,8 @@ int finish(const char* what);
 // never shrinks; freed at process exit.  Returns nullptr on failure (the
 // error is recorded).
 void* workspace(size_t bytes, int slot);
+// Bytes currently held by a workspace slot on the current device (0: none).
+size_t workspace_size(int slot);
 
 // Value types carried by descriptors (csize, in bits).
 enum class VType { F64, F32, BF16, Invalid };

@@ -31,7 +31,7 @@ struct Workspace {
   void* ptr = nullptr;
   size_t bytes = 0;
 };
-constexpr int kSlots = 8;
+constexpr int kSlots = 16;
 constexpr int kMaxDevices = 16;
 std::mutex g_ws_mutex;
 Workspace g_ws[kMaxDevices][kSlots];
@@ -121,6 +121,16 @@ void* workspace(size_t bytes, int slot) {
   w.ptr = p;
   w.bytes = want;
   return p;
+}
+
+size_t workspace_size(int slot) {
+  int dev = 0;
+  if (slot < 0 || slot >= kSlots || hipGetDevice(&dev) != hipSuccess || dev < 0 ||
+      dev >= kMaxDevices)
+    return 0;
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  const Workspace& w = g_ws[dev][slot];
+  return w.ptr ? w.bytes : 0;
 }
 
 int view_csr(const taco_tensor_t* t, const char* name, CsrView* out) {

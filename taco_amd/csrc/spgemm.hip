@@ -12,6 +12,21 @@
 //             encounter order (jA outer, kB inner), first product stored,
 //             later ones added — the order of taco's workspace loop.
 //
+// Assemble sorts each row once:
+//   1. per-row candidate counts (sum of the B row lengths A's row names) and
+//      their prefix sum give every row a scratch region that holds all of
+//      its candidates;
+//   2. one symbolic pass writes each row's sorted distinct columns into its
+//      region and its distinct count;
+//   3. the prefix sum of the distinct counts is C's pos; a compaction
+//      copies each region's sorted columns to crd.
+// Rows with more distinct columns than the LDS row set holds are listed by
+// pass 2 and sorted in global memory: their candidates are written out in
+// encounter order, sorted per row by a segmented radix sort (rocPRIM) and
+// deduplicated into their regions.  When the candidate total exceeds the
+// int32 range the two-pass protocol of taco (count, then fill, each
+// building the row set) is used instead, with the LDS row-set limit.
+//
 // Row kernels, one 64-lane wave per row, built on the row helpers of
 // include/taco_hip_rt.h (shared with modules emitted by CodeGen_HIP):
 //   short rows (<= 64 candidates): candidates in lanes, bitonic sort across
@@ -20,9 +35,14 @@
 //     encounter order and are summed sequentially in that order.
 //   long rows: a per-wave LDS hash set of up to TACO_HIP_ROW_CAP distinct
 //     columns, then an in-LDS bitonic sort; the numeric phase accumulates
-//     into LDS slots, adding candidates one at a time in encounter order.
+//     into LDS slots, adding candidates one at a time in encounter order
+//     (rows of more than TACO_HIP_ROW_CAP columns accumulate in C's values
+//     in global memory, in the same order).
 
 #include <climits>
+#include <vector>
+
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include "common.h"
 #include "device.h"
@@ -68,6 +88,149 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(
        i += stride)
     taco_hip_spgemm_row_numeric(apos, acrd, avals, bpos, bcrd, bvals, (int32_t)i,
                                 cpos, ccrd, cvals);
+}
+
+
+// ---- one-sort assemble ----
+
+// cand[i] = candidates of row i (saturated at INT_MAX; the prefix sum then
+// reports the overflow).
+__global__ __launch_bounds__(256) void k_spgemm_row_cands(
+    int32_t m, const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
+    const int32_t* __restrict__ bpos, int32_t* __restrict__ cand) {
+  const int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (i >= m) return;
+  const int64_t n = taco_hip_row_candidates(apos, acrd, bpos, (int32_t)i);
+  if (__lane_id() == 0) cand[i] = n > INT_MAX ? INT_MAX : (int32_t)n;
+}
+
+// Row i's sorted distinct columns into dst[0..count).  Returns the count,
+// or -1 (nothing written) when the row has more distinct columns than the
+// LDS row set holds.
+__device__ inline int32_t row_sorted_set(const int32_t* apos, const int32_t* acrd,
+                                         const int32_t* bpos, const int32_t* bcrd,
+                                         int32_t i, int32_t* dst) {
+  __shared__ TacoHipRowSet s_sets[kWavesPerBlock];
+  const int lane = (int)__lane_id();
+  int32_t col = INT_MAX, jA = 0, kB = 0;
+  int64_t n = taco_hip_row_lanes(apos, acrd, bpos, bcrd, i, &col, &jA, &kB);
+  if (n < 0) {
+    n = taco_hip_row_candidates(apos, acrd, bpos, i);
+    if (n <= 64 && lane < n)
+      taco_hip_row_candidate(apos, acrd, bpos, bcrd, i, lane, &col, &jA, &kB);
+  }
+  if (n <= 64) {
+    const int32_t s = taco_hip_wave_sort(col);
+    const int32_t prev = __shfl_up(s, 1, 64);
+    const bool head = s != INT_MAX && (lane == 0 || prev != s);
+    const uint64_t heads = __ballot(head);
+    if (head) dst[__popcll(heads & ((1ull << lane) - 1))] = s;
+    return __popcll(heads);
+  }
+  TacoHipRowSet* w = &s_sets[(threadIdx.x / kWave) % kWavesPerBlock];
+  const int32_t cnt = taco_hip_row_hash(w, apos, acrd, bpos, bcrd, i);
+  if (cnt < 0) return -1;
+  taco_hip_row_hash_sort(w);
+  for (int q = lane; q < cnt; q += kWave) dst[q] = w->keys[q];
+  return cnt;
+}
+
+// Pass 2.  status[0] bit 4: the scratch holds fewer entries than the
+// candidate total, or the candidate offsets overflowed (nothing done);
+// status[1]: number of rows listed in long_rows (their count is left 0 for
+// the global-memory path).
+__global__ __launch_bounds__(256) void k_spgemm_symbolic_once(
+    int32_t m, const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
+    const int32_t* __restrict__ bpos, const int32_t* __restrict__ bcrd,
+    const int32_t* __restrict__ candoff, int64_t capacity, int32_t* __restrict__ scratch,
+    int32_t* __restrict__ counts, int32_t* __restrict__ long_rows,
+    int32_t* __restrict__ status) {
+  const int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (i >= m) return;
+  // The candidate offsets overflowed int32 (status[2] bit 2: the caller
+  // falls back to the two-pass form), or the scratch is too small: write
+  // nothing.
+  if ((status[2] & 2) || (int64_t)candoff[m] > capacity) {
+    if (i == 0 && __lane_id() == 0) atomicOr(status, 4);
+    return;
+  }
+  const int32_t c = row_sorted_set(apos, acrd, bpos, bcrd, (int32_t)i, scratch + candoff[i]);
+  if (__lane_id() == 0) {
+    if (c < 0) long_rows[atomicAdd(status + 1, 1)] = (int32_t)i;
+    counts[i] = c < 0 ? 0 : c;
+  }
+}
+
+// Candidates of the listed long rows, lens[r] (candidate counts from the
+// first prefix sum).
+__global__ __launch_bounds__(256) void k_spgemm_long_lengths(
+    int32_t nlong, const int32_t* __restrict__ long_rows, const int32_t* __restrict__ candoff,
+    int32_t* __restrict__ lens) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nlong) return;
+  const int32_t i = long_rows[r];
+  lens[r] = candoff[i + 1] - candoff[i];
+}
+
+// Long row r's candidate columns, in encounter order, at keys[loff[r] ...].
+__global__ __launch_bounds__(256) void k_spgemm_long_fill(
+    int32_t nlong, const int32_t* __restrict__ long_rows, const uint32_t* __restrict__ loff,
+    const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
+    const int32_t* __restrict__ bpos, const int32_t* __restrict__ bcrd,
+    uint32_t* __restrict__ keys) {
+  const int64_t r = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (r >= nlong) return;
+  const int lane = (int)__lane_id();
+  const int32_t i = long_rows[r];
+  uint32_t o = loff[r];
+  for (int32_t p = apos[i]; p < apos[i + 1]; ++p) {
+    const int32_t j = acrd[p];
+    const int32_t b0 = bpos[j], b1 = bpos[j + 1];
+    for (int32_t kB = b0 + lane; kB < b1; kB += kWave) keys[o + (uint32_t)(kB - b0)] = (uint32_t)bcrd[kB];
+    o += (uint32_t)(b1 - b0);
+  }
+}
+
+// Long row r's sorted candidates deduplicated into its scratch region;
+// counts[i] = the distinct count.
+__global__ __launch_bounds__(256) void k_spgemm_long_dedup(
+    int32_t nlong, const int32_t* __restrict__ long_rows, const uint32_t* __restrict__ loff,
+    const uint32_t* __restrict__ sorted, const int32_t* __restrict__ candoff,
+    int32_t* __restrict__ scratch, int32_t* __restrict__ counts) {
+  const int64_t r = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (r >= nlong) return;
+  const int lane = (int)__lane_id();
+  const int32_t i = long_rows[r];
+  const uint32_t q0 = loff[r], q1 = loff[r + 1];
+  int32_t* dst = scratch + candoff[i];
+  int32_t base = 0;
+  for (uint32_t c0 = q0; c0 < q1; c0 += kWave) {
+    const uint32_t q = c0 + (uint32_t)lane;
+    bool head = false;
+    uint32_t v = 0;
+    if (q < q1) {
+      v = sorted[q];
+      head = q == q0 || sorted[q - 1] != v;
+    }
+    const uint64_t heads = __ballot(head);
+    if (head) dst[base + __popcll(heads & ((1ull << lane) - 1))] = (int32_t)v;
+    base += __popcll(heads);
+  }
+  if (lane == 0) counts[i] = base;
+}
+
+// crd: row i's sorted columns copied from its scratch region.
+__global__ __launch_bounds__(256) void k_spgemm_compact(
+    int32_t m, const int32_t* __restrict__ candoff, const int32_t* __restrict__ cpos,
+    const int32_t* __restrict__ scratch, int32_t* __restrict__ ccrd) {
+  const int64_t stride = (int64_t)gridDim.x * kWavesPerBlock;
+  const int lane = (int)__lane_id();
+  for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; i < m;
+       i += stride) {
+    const int32_t c0 = cpos[i], n = cpos[i + 1] - c0;
+    const int32_t* src = scratch + candoff[i];
+    for (int32_t q = lane; q < n; q += kWave) __builtin_nontemporal_store(src[q], ccrd + c0 + q);
+  }
 }
 
 
@@ -189,11 +352,20 @@ int check_spgemm_operands(taco_tensor_t* C, taco_tensor_t* A, taco_tensor_t* B,
 
 }  // namespace
 
-extern "C" int taco_hip_spgemm_assemble(taco_tensor_t* C, taco_tensor_t* A,
-                                        taco_tensor_t* B) {
-  CsrView a, b;
-  if (int rc = check_spgemm_operands(C, A, B, &a, &b)) return rc;
-  hipStream_t st = current_stream();
+namespace {
+
+void publish(taco_tensor_t* C, int32_t* cpos, int32_t* ccrd, double* cvals, int32_t nnz) {
+  C->indices[1][0] = (uint8_t*)cpos;
+  C->indices[1][1] = (uint8_t*)ccrd;
+  C->vals = (uint8_t*)cvals;
+  C->vals_size = nnz;
+}
+
+// taco's two-phase protocol as it stands: a count pass and a fill pass, each
+// building the row set (rows limited to kHashCap distinct columns).  Used
+// when the candidate total exceeds the int32 range of the one-sort scratch.
+int assemble_two_pass(const CsrView& a, const CsrView& b, taco_tensor_t* C,
+                      hipStream_t st) {
   const int32_t m = a.m;
   int32_t* cpos = (int32_t*)taco_hip_unified_alloc(sizeof(int32_t) * ((size_t)m + 1));
   if (!cpos) return 1;
@@ -227,7 +399,8 @@ extern "C" int taco_hip_spgemm_assemble(taco_tensor_t* C, taco_tensor_t* A,
     }
     if (host[1] & 1) {
       rc = fail("spgemm: a row of C has more than %d distinct columns, the "
-                "limit of the LDS row kernel", kHashCap);
+                "limit of the LDS row kernel (two-pass form, used when the "
+                "candidate total exceeds the int32 range)", kHashCap);
       break;
     }
     if (host[1] & 2) {
@@ -251,14 +424,191 @@ extern "C" int taco_hip_spgemm_assemble(taco_tensor_t* C, taco_tensor_t* A,
       taco_hip_unified_free(cvals);
       break;
     }
-    C->indices[1][0] = (uint8_t*)cpos;
-    C->indices[1][1] = (uint8_t*)ccrd;
-    C->vals = (uint8_t*)cvals;
-    C->vals_size = nnz;
+    publish(C, cpos, ccrd, cvals, nnz);
     return 0;
   } while (0);
   taco_hip_unified_free(cpos);
   return rc;
+}
+
+// Workspace slots of the one-sort assemble.
+constexpr int kSlotCandOff = 8, kSlotScratch = 9, kSlotLongMeta = 10, kSlotLongKeys = 11,
+              kSlotSortTemp = 12;
+
+// Rows listed by the symbolic pass (more than kHashCap distinct columns):
+// candidates written out, sorted per row in global memory, deduplicated into
+// their scratch regions with their distinct counts.
+int assemble_long_rows(const CsrView& a, const CsrView& b, int32_t nlong,
+                       const int32_t* long_rows, const int32_t* candoff, int32_t* scratch,
+                       int32_t* counts, hipStream_t st) {
+  char* meta = (char*)workspace(sizeof(uint32_t) * (2 * (size_t)nlong + 2), kSlotLongMeta);
+  if (!meta) return 1;
+  int32_t* lens = (int32_t*)meta;
+  uint32_t* loff = (uint32_t*)meta + nlong + 1;
+  k_spgemm_long_lengths<<<(nlong + 255) / 256, 256, 0, st>>>(nlong, long_rows, candoff, lens);
+  TH_LAUNCH_CHECK("spgemm long rows");
+  std::vector<int32_t> h_lens((size_t)nlong);
+  TH_CHECK(hipMemcpyAsync(h_lens.data(), lens, sizeof(int32_t) * (size_t)nlong,
+                          hipMemcpyDeviceToHost, st));
+  TH_CHECK(hipStreamSynchronize(st));
+  std::vector<uint32_t> h_off((size_t)nlong + 1, 0);
+  uint64_t total = 0;
+  for (int32_t r = 0; r < nlong; ++r) {
+    h_off[(size_t)r] = (uint32_t)total;
+    total += (uint64_t)h_lens[(size_t)r];
+  }
+  TH_REQUIRE(total < 0xFFFFFFFFull, "spgemm: long rows hold too many candidates");
+  h_off[(size_t)nlong] = (uint32_t)total;
+  TH_CHECK(hipMemcpyAsync(loff, h_off.data(), sizeof(uint32_t) * ((size_t)nlong + 1),
+                          hipMemcpyHostToDevice, st));
+  uint32_t* keys = (uint32_t*)workspace(sizeof(uint32_t) * 2 * (size_t)(total ? total : 1),
+                                        kSlotLongKeys);
+  if (!keys) return 1;
+  uint32_t* sorted = keys + total;
+  const unsigned grid = (unsigned)(((int64_t)nlong + kWavesPerBlock - 1) / kWavesPerBlock);
+  k_spgemm_long_fill<<<grid, 256, 0, st>>>(nlong, long_rows, loff, a.pos, a.crd, b.pos, b.crd,
+                                           keys);
+  TH_LAUNCH_CHECK("spgemm long rows: candidates");
+  unsigned bits = 1;
+  while (bits < 32 && (1ull << bits) < (uint64_t)(b.n > 1 ? b.n : 2)) ++bits;
+  size_t temp_bytes = 0;
+  TH_CHECK(rocprim::segmented_radix_sort_keys(nullptr, temp_bytes, keys, sorted,
+                                              (unsigned)total, (unsigned)nlong, loff, loff + 1,
+                                              0, bits, st));
+  void* temp = workspace(temp_bytes ? temp_bytes : 1, kSlotSortTemp);
+  if (!temp) return 1;
+  TH_CHECK(rocprim::segmented_radix_sort_keys(temp, temp_bytes, keys, sorted, (unsigned)total,
+                                              (unsigned)nlong, loff, loff + 1, 0, bits, st));
+  k_spgemm_long_dedup<<<grid, 256, 0, st>>>(nlong, long_rows, loff, sorted, candoff, scratch,
+                                            counts);
+  TH_LAUNCH_CHECK("spgemm long rows: distinct columns");
+  return 0;
+}
+
+// The one-sort assemble (see the file comment).  *two_pass is set when the
+// candidate total exceeds the int32 range; nothing is published then.
+int assemble_one_sort(const CsrView& a, const CsrView& b, taco_tensor_t* C, hipStream_t st,
+                      bool* two_pass) {
+  *two_pass = false;
+  const int32_t m = a.m;
+  // status[0]: flags (2: nnz(C) beyond int32, 4: scratch too small);
+  // status[1]: long rows listed; status[2]: candidate-total flags (2).
+  char* ws = (char*)workspace(256 + sizeof(int32_t) * 2 * ((size_t)m + 1), 2);
+  int32_t* candoff = (int32_t*)workspace(sizeof(int32_t) * ((size_t)m + 1), kSlotCandOff);
+  if (!ws || !candoff) return 1;
+  int32_t* status = (int32_t*)ws;
+  int32_t* counts = (int32_t*)(ws + 256);
+  int32_t* long_rows = counts + m + 1;
+  int32_t* cpos = (int32_t*)taco_hip_unified_alloc(sizeof(int32_t) * ((size_t)m + 1));
+  if (!cpos) return 1;
+  const unsigned grid = (unsigned)(((int64_t)m + kWavesPerBlock - 1) / kWavesPerBlock);
+  int rc = 0;
+  do {
+    if (hipMemsetAsync(status, 0, 4 * sizeof(int32_t), st) != hipSuccess) {
+      rc = fail("spgemm: memset failed");
+      break;
+    }
+    if (m > 0)
+      k_spgemm_row_cands<<<grid, 256, 0, st>>>(m, a.pos, a.crd, b.pos, counts);
+    if ((rc = device_counts_to_pos(m, counts, candoff, status + 2, st))) break;
+    size_t cap_bytes = workspace_size(kSlotScratch);
+    int32_t* scratch = (int32_t*)workspace(cap_bytes ? cap_bytes : 4096, kSlotScratch);
+    if (!scratch) {
+      rc = 1;
+      break;
+    }
+    cap_bytes = workspace_size(kSlotScratch);
+    int32_t host[5] = {0, 0, 0, 0, 0};  // status[0..2], candoff[m], cpos[m]
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      if (hipMemsetAsync(status, 0, 2 * sizeof(int32_t), st) != hipSuccess) {
+        rc = fail("spgemm: memset failed");
+        break;
+      }
+      if (m > 0)
+        k_spgemm_symbolic_once<<<grid, 256, 0, st>>>(
+            m, a.pos, a.crd, b.pos, b.crd, candoff, (int64_t)(cap_bytes / sizeof(int32_t)),
+            scratch, counts, long_rows, status);
+      if ((rc = device_counts_to_pos(m, counts, cpos, status, st))) break;
+      if (hipMemcpyAsync(host, status, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
+              hipSuccess ||
+          hipMemcpyAsync(&host[3], candoff + m, sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
+              hipSuccess ||
+          hipMemcpyAsync(&host[4], cpos + m, sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
+              hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess) {
+        rc = fail("spgemm: reading nnz(C) failed");
+        break;
+      }
+      if ((host[2] & 2) || !(host[0] & 4)) break;
+      // The scratch was too small for the candidate total: grow, run again.
+      cap_bytes = sizeof(int32_t) * (size_t)host[3];
+      scratch = (int32_t*)workspace(cap_bytes, kSlotScratch);
+      if (!scratch) {
+        rc = 1;
+        break;
+      }
+    }
+    if (rc) break;
+    if (host[2] & 2) {
+      *two_pass = true;
+      break;
+    }
+    TH_REQUIRE(!(host[0] & 4), "spgemm: scratch sizing failed");
+    if (host[1] > 0) {
+      if ((rc = assemble_long_rows(a, b, host[1], long_rows, candoff, scratch, counts, st)))
+        break;
+      if (hipMemsetAsync(status, 0, sizeof(int32_t), st) != hipSuccess) {
+        rc = fail("spgemm: memset failed");
+        break;
+      }
+      if ((rc = device_counts_to_pos(m, counts, cpos, status, st))) break;
+      if (hipMemcpyAsync(host, status, sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
+              hipSuccess ||
+          hipMemcpyAsync(&host[4], cpos + m, sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
+              hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess) {
+        rc = fail("spgemm: reading nnz(C) failed");
+        break;
+      }
+    }
+    if (host[0] & 2) {
+      rc = fail("spgemm: nnz(C) exceeds the int32 index range");
+      break;
+    }
+    const int32_t nnz = host[4];
+    int32_t* ccrd = (int32_t*)taco_hip_unified_alloc(sizeof(int32_t) * (size_t)nnz);
+    double* cvals = (double*)taco_hip_unified_alloc(sizeof(double) * (size_t)nnz);
+    if (nnz > 0 && (!ccrd || !cvals)) {
+      taco_hip_unified_free(ccrd);
+      taco_hip_unified_free(cvals);
+      rc = 1;
+      break;
+    }
+    if (m > 0 && nnz > 0)
+      k_spgemm_compact<<<(unsigned)taco_hip_row_grid(m), 256, 0, st>>>(m, candoff, cpos,
+                                                                        scratch, ccrd);
+    if ((rc = finish("taco_hip_spgemm_assemble"))) {
+      taco_hip_unified_free(ccrd);
+      taco_hip_unified_free(cvals);
+      break;
+    }
+    publish(C, cpos, ccrd, cvals, nnz);
+    return 0;
+  } while (0);
+  taco_hip_unified_free(cpos);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int taco_hip_spgemm_assemble(taco_tensor_t* C, taco_tensor_t* A,
+                                        taco_tensor_t* B) {
+  CsrView a, b;
+  if (int rc = check_spgemm_operands(C, A, B, &a, &b)) return rc;
+  hipStream_t st = current_stream();
+  bool two_pass = false;
+  if (int rc = assemble_one_sort(a, b, C, st, &two_pass)) return rc;
+  return two_pass ? assemble_two_pass(a, b, C, st) : 0;
 }
 
 extern "C" int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,

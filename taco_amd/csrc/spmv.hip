@@ -26,6 +26,7 @@
 //                   are deterministic run to run.
 
 #include <climits>
+#include <mutex>
 
 #include "common.h"
 #include "device.h"
@@ -284,8 +285,11 @@ __global__ __launch_bounds__(256) void k_spmv_merge_fixup(
   y[r] += s;
 }
 
+// ext_scratch / ext_bytes: caller-provided scratch for the merge path's
+// carries (the no-allocation entry point); nullptr: the library workspace.
 template <typename T>
-int run_spmv(const CsrView& a, const T* x, T* y, const taco_hip_schedule& s) {
+int run_spmv(const CsrView& a, const T* x, T* y, const taco_hip_schedule& s,
+             void* ext_scratch = nullptr, size_t ext_bytes = 0) {
   hipStream_t st = current_stream();
   if (a.m == 0) return 0;
   const int algo = s.algo;
@@ -347,9 +351,17 @@ int run_spmv(const CsrView& a, const T* x, T* y, const taco_hip_schedule& s) {
     const int64_t ipw = 256 * (int64_t)k;
     const int64_t nblocks = (total + ipw - 1) / ipw;
     TH_REQUIRE(nblocks < INT32_MAX, "spmv: problem too large");
-    void* ws = workspace((size_t)nblocks * (sizeof(int32_t) + sizeof(T)) + 64,
-                         0);
-    if (!ws) return 1;
+    const size_t need = (size_t)nblocks * (sizeof(int32_t) + sizeof(T)) + 64;
+    void* ws = ext_scratch;
+    if (ws) {
+      TH_REQUIRE(need <= ext_bytes,
+                 "spmv (no-allocation form): the merge path needs %zu bytes of scratch "
+                 "for %lld rows + entries at %d items per thread; the limit is %zu bytes",
+                 need, (long long)total, k, ext_bytes);
+    } else {
+      ws = workspace(need, 0);
+      if (!ws) return 1;
+    }
     int32_t* crow = (int32_t*)ws;
     T* cval = (T*)((char*)ws + ((size_t)nblocks * sizeof(int32_t) + 15) / 16 * 16);
 #define TH_MP(K)                                                           \
@@ -377,9 +389,38 @@ int run_spmv(const CsrView& a, const T* x, T* y, const taco_hip_schedule& s) {
 
 using namespace taco_hip;
 
-extern "C" int taco_hip_spmv_compute(taco_tensor_t* y, taco_tensor_t* A,
-                                     taco_tensor_t* x,
-                                     const taco_hip_schedule* sched) {
+namespace {
+
+// Scratch of the no-allocation entry point: the merge path's per-workgroup
+// carries (row, partial sum), written by every workgroup before the fix-up
+// kernel reads them, so nothing needs clearing between calls.  Each device
+// holds its own copy until the process ends; host code uses the address
+// hipGetSymbolAddress returns for the current device, never the symbol.
+constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kScratchDevices] = {};
+
+// The current device's scratch address, resolved once per device.
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  TH_CHECK(hipGetDevice(&dev));
+  TH_REQUIRE(dev >= 0 && dev < kScratchDevices, "spmv (no-allocation form): device %d", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    TH_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TH_REQUIRE(p != nullptr, "spmv (no-allocation form): no scratch address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+int spmv_entry(taco_tensor_t* y, taco_tensor_t* A, taco_tensor_t* x,
+               const taco_hip_schedule* sched, bool no_alloc) {
   CsrView a;
   DenseView xv, yv;
   if (int rc = view_csr(A, "A", &a)) return rc;
@@ -396,13 +437,31 @@ extern "C" int taco_hip_spmv_compute(taco_tensor_t* y, taco_tensor_t* A,
     s = *sched;
   else
     taco_hip_default_schedule(0, a.m, a.nnz, 1, &s);
+  void* ext = nullptr;
+  if (no_alloc)
+    if (int rc = noalloc_scratch(&ext)) return rc;
+  const size_t ext_bytes = no_alloc ? kNoAllocScratchBytes : 0;
   int rc;
   if (a.vt == VType::F64)
-    rc = run_spmv<double>(a, (const double*)xv.vals, (double*)yv.vals, s);
+    rc = run_spmv<double>(a, (const double*)xv.vals, (double*)yv.vals, s, ext, ext_bytes);
   else if (a.vt == VType::F32)
-    rc = run_spmv<float>(a, (const float*)xv.vals, (float*)yv.vals, s);
+    rc = run_spmv<float>(a, (const float*)xv.vals, (float*)yv.vals, s, ext, ext_bytes);
   else
     return fail("spmv: value type must be f64 or f32");
   if (rc) return rc;
-  return finish("taco_hip_spmv_compute");
+  return finish(no_alloc ? "taco_hip_spmv_compute_noalloc" : "taco_hip_spmv_compute");
+}
+
+}  // namespace
+
+extern "C" int taco_hip_spmv_compute(taco_tensor_t* y, taco_tensor_t* A,
+                                     taco_tensor_t* x,
+                                     const taco_hip_schedule* sched) {
+  return spmv_entry(y, A, x, sched, false);
+}
+
+extern "C" int taco_hip_spmv_compute_noalloc(taco_tensor_t* y, taco_tensor_t* A,
+                                             taco_tensor_t* x,
+                                             const taco_hip_schedule* sched) {
+  return spmv_entry(y, A, x, sched, true);
 }

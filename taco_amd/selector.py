@@ -45,10 +45,15 @@ DEFAULT_CLASS = {0: SPMV_NAMES.index("thread_per_row"),
                  1: N_SPMV + SPMM_NAMES.index("thread_per_row")}
 
 
-# Classes whose kernels accumulate with atomics (the nnz-split schedules):
-# their results can differ in the last bits from run to run, always within
-# the value tolerance.  Every other class sums in a fixed order.
-ATOMIC_CLASSES = {c for c, name in enumerate(CLASS_NAMES) if "nnz_split" in name}
+# Classes whose kernels accumulate with atomics (the nnz-split schedules and
+# the slab plan's LDS row partials): their results can differ in the last
+# bits from run to run, always within the value tolerance.  Every other
+# class sums in a fixed order.
+ATOMIC_CLASSES = {c for c, name in enumerate(CLASS_NAMES)
+                  if "nnz_split" in name or name == "slab_plan"}
+# The slab plan (a storage conversion of A, reused across calls): its class
+# time is the per-call time plus build time / autotune.PLAN_CALLS.
+PLAN_CLASS = SPMV_NAMES.index("slab_plan")
 
 
 def class_range(op):
@@ -56,7 +61,8 @@ def class_range(op):
 
 
 def class_schedule(c, K=0):
-    """The taco_hip_schedule of class c (SpMM group sizes follow K)."""
+    """The taco_hip_schedule of class c (SpMM group sizes follow K); None
+    for the slab plan."""
     if c < N_SPMV:
         return autotune.SPMV_CANDIDATES[c][1]
     return autotune.spmm_candidates(K)[c - N_SPMV][1]
@@ -223,10 +229,16 @@ class DeviceProblem:
         self._keep = (pos, crd, vals)
 
     def runner(self, c):
+        """The call that runs class c (None: the slab plan does not apply to
+        this matrix).  The slab plan is built on first use; its build time
+        is self.plan_build_ms."""
         L = _lib.lib()
         s = class_schedule(c, self.K)
         if self.op == 0:
-            return lambda: _lib.check(L.taco_hip_spmv_compute(self.cd.ptr, self.a.ptr, self.bd.ptr, s))
+            if s is None and not hasattr(self, "plan"):
+                self.plan, self.plan_build_ms = autotune.slab_plan(self.a)
+            return autotune.spmv_runner(s, self.a, self.bd, self.cd,
+                                        getattr(self, "plan", None))
         return lambda: _lib.check(L.taco_hip_spmm_compute(self.cd.ptr, self.a.ptr, self.bd.ptr, s))
 
 
@@ -252,6 +264,8 @@ def time_classes(prob, reps=3, slow_factor=4.0, rtol=None):
     best = np.inf
     for c in list(range(lo + 1, hi)) + [lo]:
         fn = prob.runner(c)
+        if fn is None:
+            continue
         if c != lo:
             fn()
             torch.cuda.synchronize()
@@ -272,6 +286,8 @@ def time_classes(prob, reps=3, slow_factor=4.0, rtol=None):
             torch.cuda.synchronize()
             ts = [t1] + [e[r].elapsed_time(e[r + 1]) for r in range(1, reps)]
             times[c] = float(np.median(ts))
+        if c == PLAN_CLASS and prob.op == 0:
+            times[c] += prob.plan_build_ms / autotune.PLAN_CALLS
         best = min(best, times[c])
     return times
 

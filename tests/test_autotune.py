@@ -31,7 +31,16 @@ def test_rejected_candidate_ranks_last(monkeypatch):
 
     monkeypatch.setattr(autotune, "_time", lambda fn, reps: timed.append(fn) or 1.0)
     monkeypatch.setattr(torch.cuda, "synchronize", lambda: None)
-    runs = [("ref", run(2.0)), ("same", run(2.0)), ("wrong", run(2.5))]
+    runs = [("ref", run(2.0), 0.0), ("same", run(2.0), 0.25), ("wrong", run(2.5), 0.0),
+            ("unavailable", None, 0.0)]
     res = autotune._tune(runs, out, 3, 1e-10)
-    assert [n for n, _ in res] == ["ref", "same", "wrong"]
-    assert res[-1][1] == math.inf and len(timed) == 2
+    assert [n for n, _ in res] == ["ref", "same", "wrong", "unavailable"]
+    assert res[1][1] == 1.25  # the extra per-call cost (a plan's build share) is added
+    assert res[-1][1] == math.inf and res[-2][1] == math.inf and len(timed) == 2
+
+
+def test_plan_candidate_replaces_the_sixteen_entry_split():
+    names = [n for n, _ in autotune.SPMV_CANDIDATES]
+    assert len(names) == 13 and "slab_plan" in names and "nnz_split16" not in names
+    assert dict(autotune.SPMV_CANDIDATES)["slab_plan"] is None
+    assert autotune.PLAN_CALLS == 100

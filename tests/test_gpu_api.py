@@ -122,3 +122,62 @@ def test_f32_spmv_through_the_api():
     A64 = W.Csr(3000, 2500, Ah.pos, Ah.crd, Ah.vals.astype(np.float32).astype(np.float64))
     ye, cond = oracle.exact_spmv(A64, xh.astype(np.float32).astype(np.float64))
     assert oracle.error_report(y.to_numpy(), ye, cond, 1e-5)["ok"]
+
+
+needs_weights = pytest.mark.skipif(
+    not os.path.exists(os.path.join(os.path.dirname(taco.__file__), "data", "selector.npz")),
+    reason="no trained selector weights in taco_amd/data")
+
+
+@needs_weights
+def test_learned_schedule_through_compute():
+    """set_learned_schedule_enabled: an unscheduled SpMV and SpMM get the
+    selector's schedule at compute(); the slab plan is kept on A and dropped
+    when A's values change; every result passes the FP64 oracle."""
+    from taco_amd import selector as Sel
+    i, j, k = taco.IndexVar("i"), taco.IndexVar("j"), taco.IndexVar("k")
+    Ah = W.csr_powerlaw(60000, 60000, seed=54, kmax=20000)
+    xh = W.uniform(Ah.n, 54, W.STREAM_X)
+    taco.set_learned_schedule_enabled(True)
+    try:
+        A = csr_tensor("A", [Ah.m, Ah.n], Ah.pos, Ah.crd, Ah.vals, insert=False)
+        x = taco.Tensor("x", [Ah.n], taco.Format([taco.Dense]))
+        x.set_values(xh)
+        y = taco.Tensor("y", [Ah.m], taco.Format([taco.Dense]))
+        y[i] = A[i, j] * x[j]
+        y.compile()
+        y.compute()
+        picked = y._kernel.chosen()
+        print(f"learned SpMV schedule: {picked}", flush=True)
+        assert picked in Sel.CLASS_NAMES[:Sel.N_SPMV]
+        ye, cond = oracle.exact_spmv(Ah, xh)
+        assert oracle.error_report(y.to_numpy(), ye, cond, 1e-12)["ok"]
+        # the slab plan through compute(): built once, reused, then dropped
+        A._schedule_choice[(0, 0)] = (Sel.PLAN_CLASS, None)
+        y.compute()
+        plan = A._plan
+        assert plan is not None
+        assert oracle.error_report(y.to_numpy(), ye, cond, 1e-12)["ok"]
+        y.compute()
+        assert A._plan is plan
+        vals2 = -2.0 * Ah.vals
+        A.set_csr(Ah.pos, Ah.crd, vals2)
+        assert A._plan is None and A._schedule_choice == {}
+        A._schedule_choice[(0, 0)] = (Sel.PLAN_CLASS, None)
+        y.compute()
+        assert A._plan is not None and A._plan is not plan
+        ye2, cond2 = oracle.exact_spmv(W.Csr(Ah.m, Ah.n, Ah.pos, Ah.crd, vals2), xh)
+        assert oracle.error_report(y.to_numpy(), ye2, cond2, 1e-12)["ok"]
+        # SpMM, K = 32
+        Bh = W.uniform(Ah.n * 32, 55, W.STREAM_B).reshape(Ah.n, 32)
+        B = taco.Tensor("B", [Ah.n, 32], taco.dense_matrix)
+        B.set_values(Bh)
+        C = taco.Tensor("C", [Ah.m, 32], taco.dense_matrix)
+        C[i, k] = A[i, j] * B[j, k]
+        C.compile()
+        C.compute()
+        assert C._kernel.chosen(32) in Sel.CLASS_NAMES[Sel.N_SPMV:]
+        Ce, condc = oracle.exact_spmm(W.Csr(Ah.m, Ah.n, Ah.pos, Ah.crd, vals2), Bh)
+        assert oracle.error_report(C.to_numpy(), Ce, condc, 1e-12)["ok"]
+    finally:
+        taco.set_learned_schedule_enabled(False)

@@ -203,6 +203,13 @@ def spgemm_cases():
     counts[np.random.default_rng(30).random(2500) < 0.4] = 0
     counts[::97] = 70
     yield "gappy_b", W.csr_fixed(3000, 2500, 6, 28), W.csr_from_counts(2500, 3000, counts, 31)
+    # C rows of more than 1,024 distinct columns (beyond the LDS row set):
+    # sorted in global memory when assembling, accumulated in C's values
+    # when computing; mixed with short, hashed and empty rows
+    ca = np.random.default_rng(32).integers(0, 12, 2000).astype(np.int32)
+    ca[::101] = 1500
+    ca[5::257] = 0
+    yield "over_row_cap", W.csr_from_counts(2000, 30000, ca, 33), W.csr_fixed(30000, 40000, 4, 34)
 
 
 @pytest.mark.parametrize("name,A,B", list(spgemm_cases()), ids=lambda v: v if isinstance(v, str) else "")
@@ -214,6 +221,8 @@ def test_spgemm(name, A, B):
     ex, cond = oracle.exact_spgemm_values(A, B, opos, ocrd)
     rep = oracle.error_report(vals, ex, cond, TOL["f64"])
     assert rep["ok"], rep
+    if name == "over_row_cap":
+        assert np.diff(pos).max() > 4000  # the global-memory row path ran
     if name in ("fixed8", "long_a_empty_b"):
         # Short rows sum duplicates sequentially in taco's encounter order,
         # with separately rounded products: equal to taco's values bit for bit.

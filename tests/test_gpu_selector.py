@@ -8,6 +8,7 @@ import torch
 
 import oracle
 from oracle import selector_ref as R
+from taco_amd import autotune
 from taco_amd import _lib, selector as S
 from taco_amd import descriptors as D
 from taco_amd import workloads as W
@@ -121,8 +122,12 @@ def test_selected_schedule_changes_no_result():
         got = prob.out.cpu().numpy().astype(np.float64)
         prob.out.fill_(float("nan"))
         L = _lib.lib()
-        fn = L.taco_hip_spmv_compute if op == 0 else L.taco_hip_spmm_compute
-        _lib.check(fn(prob.cd.ptr, prob.a.ptr, prob.bd.ptr, sched))
+        if sched is None:  # the slab plan, built again
+            plan, _ = autotune.slab_plan(prob.a)
+            plan.compute(prob.cd, prob.bd)
+        else:
+            fn = L.taco_hip_spmv_compute if op == 0 else L.taco_hip_spmm_compute
+            _lib.check(fn(prob.cd.ptr, prob.a.ptr, prob.bd.ptr, sched))
         torch.cuda.synchronize()
         again = prob.out.cpu().numpy().astype(np.float64)
         if c not in S.ATOMIC_CLASSES:  # fixed summation order: bit for bit
