@@ -343,6 +343,27 @@ def _recognise(stmt, kind, a):
     raise TacoError(f"schedule not supported by the HIP backend: {ops}")
 
 
+# ------------------------------------------------------ backend toggles --
+# pytaco exposes taco's GPU backend switch (reference python_bindings/src/
+# pytaco.cpp:14, should_use_cuda_codegen; include/taco/cuda.h:15-28); the
+# facade forwards the same switch to the HIP runtime.  The facade compiles
+# through CodeGen_HIP only: with code generation off, compile() raises.
+
+def should_use_cuda_codegen():
+    """taco's should_use_CUDA_codegen(): whether statements compile to GPU
+    code (here CodeGen_HIP)."""
+    return bool(_lib.lib().taco_hip_should_use_codegen())
+
+
+def set_cuda_codegen_enabled(enabled):
+    """taco's set_CUDA_codegen_enabled()."""
+    _lib.lib().taco_hip_set_codegen_enabled(1 if enabled else 0)
+
+
+should_use_hip_codegen = should_use_cuda_codegen
+set_hip_codegen_enabled = set_cuda_codegen_enabled
+
+
 # ----------------------------------------------------- learned schedules --
 
 _learned = {"enabled": False, "selector": None}
@@ -532,6 +553,9 @@ class Tensor:
 
     # -- compile / assemble / compute
     def compile(self, stmt=None):
+        if not should_use_cuda_codegen():
+            raise TacoError("GPU code generation is off (set_cuda_codegen_enabled); the "
+                            "facade compiles through CodeGen_HIP only")
         stmt = stmt if stmt is not None else self.get_assignment().concretize()
         if stmt.assignment.lhs.tensor is not self:
             raise TacoError(f"statement {stmt!r} does not assign {self.name}")

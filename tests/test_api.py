@@ -163,3 +163,20 @@ def test_equals_merge_skips_stored_zeros():
     assert taco.equals(e, b)
     f = _csr("f", [2, 3], [0, 1, 3], [0, 1, 2], [5.0, 7.0, 3.0])
     assert not taco.equals(f, b)
+
+
+def test_codegen_toggle_as_pytaco_exposes_it():
+    """should_use_cuda_codegen / set_cuda_codegen_enabled (pytaco.cpp:14,
+    cuda.h:15-28) forward to the HIP runtime; with GPU code generation off
+    the facade, which has no CPU code path, refuses to compile."""
+    was = taco.should_use_cuda_codegen()
+    try:
+        A, x, y, i, j, pre = spmv_setup()
+        taco.set_cuda_codegen_enabled(False)
+        assert not taco.should_use_cuda_codegen() and not taco.should_use_hip_codegen()
+        with pytest.raises(taco.TacoError, match="code generation is off"):
+            y.compile()
+        taco.set_hip_codegen_enabled(True)
+        assert taco.should_use_cuda_codegen()
+    finally:
+        taco.set_cuda_codegen_enabled(was)
