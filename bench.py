@@ -333,7 +333,7 @@ def bench_spmv(args, A, x, name, workload_desc, spmm=None):
                "kind": kind, "ms": cms, "median_ms": float(np.median(cpu_ts)),
                "sample": f"whole workload, {len(cpu_ts)} timed compute() calls "
                          f"(10% trimmed mean), JIT excluded, inputs in host DRAM"}
-    traffic = profiled_traffic("k_slab_spmv_wave", name)
+    traffic = profiled_traffic("k_slab_spmv", name)
     kernel_gbs = nbytes / (kernel_ms * 1e-3) / 1e9
     spmm_legs = None
     if spmm is not None:
@@ -353,8 +353,8 @@ def bench_spmv(args, A, x, name, workload_desc, spmm=None):
         "dtype": "f64", "data": "synthetic (seed-generated)",
         "config": {"workload": name, "desc": workload_desc, "m": A.m, "n": A.n,
                    "nnz": A.nnz, "schedule": f"slab-tiled plan parts={plan_info['parts']} "
-                                             f"slab_width={plan_info['slab_width']} unroll={args.unroll or 1} "
-                                             f"threads=256 wave_reduce=1",
+                                             f"slab_width={plan_info['slab_width']} unroll={args.unroll or 4} "
+                                             f"threads=256, segments ordered by column",
                    "parallelism": "single GPU"},
         "pct_of_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS,
         "gflops": 2.0 * A.nnz / (ms * 1e-3) / 1e9,
@@ -368,7 +368,7 @@ def bench_spmv(args, A, x, name, workload_desc, spmm=None):
         "cold": cold,
         "roofline": {"bound": "hbm", "achieved": kernel_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kernel_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_slab_spmv_wave", "kernel_us": kernel_ms * 1e3,
+                     "kernel": "k_slab_spmv", "kernel_us": kernel_ms * 1e3,
                      "kernel_launches_timed": kernel_launches,
                      "algorithmic_bytes_per_launch": nbytes},
         "error_vs_fp64_oracle": errors,
@@ -805,30 +805,31 @@ def gather_ceiling(args):
     out = (ctypes.c_double * 3)()
     reps = max(3, min(args.steps, 20))
     rows = []
-    for log2 in (11, 14, 17, 21, 25):
-        for mode, unrolls, wgs in ((0, (1, 4, 8, 16), (4, 8)), (1, (4, 16), (8,)),
-                                   (2, (4, 8), (8,))):
-            for u in unrolls:
-                for wg in wgs:
-                    log(f"gather ceiling: footprint 2^{log2} f64, mode {mode}, "
-                        f"unroll {u}, {wg} workgroups/CU")
-                    _lib.check(L.taco_hip_gather_ceiling(mode, u, log2, wg, POWERLAW_GATHERS,
-                                                         reps, out), "gather ceiling")
-                    ms, lane_loads, wave_loads = out[0], out[1], out[2]
-                    loads = lane_loads + wave_loads
-                    rate = loads / (ms * 1e-3)
-                    rows.append({"footprint_bytes": 8 << log2,
-                                 "path": ("vector", "scalar", "mixed")[mode],
-                                 "unroll": u, "wg_per_cu": wg, "ms": ms,
-                                 "lane_loads": lane_loads, "wave_loads": wave_loads,
-                                 "gloads_per_s": rate / 1e9,
-                                 "powerlaw_gathers_us": POWERLAW_GATHERS / rate * 1e6})
+    if args.ceiling_points:
+        # log2:mode:unroll:wg_per_cu points only (for counter runs)
+        points = [tuple(int(v) for v in p.split(":")) for p in args.ceiling_points.split(",")]
+    else:
+        points = [(log2, mode, u, wg) for log2 in (11, 14, 17, 21, 25)
+                  for mode, unrolls, wgs in ((0, (1, 4, 8, 16), (4, 8)), (1, (4, 16), (8,)),
+                                             (2, (4, 8), (8,)))
+                  for u in unrolls for wg in wgs]
+    for log2, mode, u, wg in points:
+        log(f"gather ceiling: footprint 2^{log2} f64, mode {mode}, unroll {u}, "
+            f"{wg} workgroups/CU")
+        _lib.check(L.taco_hip_gather_ceiling(mode, u, log2, wg, POWERLAW_GATHERS, reps, out),
+                   "gather ceiling")
+        ms, lane_loads, wave_loads = out[0], out[1], out[2]
+        rate = (lane_loads + wave_loads) / (ms * 1e-3)
+        rows.append({"footprint_bytes": 8 << log2, "path": ("vector", "scalar", "mixed")[mode],
+                     "unroll": u, "wg_per_cu": wg, "ms": ms, "lane_loads": lane_loads,
+                     "wave_loads": wave_loads, "gloads_per_s": rate / 1e9,
+                     "powerlaw_gathers_us": POWERLAW_GATHERS / rate * 1e6})
     best = {}
     for r in rows:
         k = (r["footprint_bytes"], r["path"])
         if k not in best or r["gloads_per_s"] > best[k]["gloads_per_s"]:
             best[k] = r
-    l2 = best[(8 << 17, "vector")]
+    l2 = best.get((8 << 17, "vector")) or max(rows, key=lambda r: r["gloads_per_s"])
     return {
         "metric": "random 8-byte gather rate (G loads/s), 1 MB footprint, vector path",
         "value": l2["gloads_per_s"], "unit": "G loads/s", "n_gpus": 1, "steps": reps,
@@ -867,7 +868,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="spmv_powerlaw")
     ap.add_argument("--items", type=int, default=8, help="SpMV merge items per thread")
-    ap.add_argument("--parts", type=int, default=1024, help="slab plan parts")
+    ap.add_argument("--parts", type=int, default=0, help="slab plan parts (0 = library default)")
     ap.add_argument("--unroll", type=int, default=0, help="slab plan entries per thread per step (0 = library default)")
     ap.add_argument("--spmm-items", type=int, default=256)
     ap.add_argument("--mfma", type=int, default=1)
@@ -879,6 +880,8 @@ def main():
     ap.add_argument("--batched-sched", type=int, default=None,
                     help="batched workloads: one lane count for every instance, no autotuning")
     ap.add_argument("--problems", type=int, default=200, help="selector held-out problems")
+    ap.add_argument("--ceiling-points", default="",
+                    help="gather_ceiling: only these log2:mode:unroll:wg_per_cu points")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "

@@ -182,9 +182,11 @@ int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,
 
 /* Slab-tiled SpMV plan (f64).  A storage conversion of a CSR matrix for
  * SpMV whose x exceeds one XCD's L2: columns split into slabs of slab_width
- * (<= 2^18; 0 = 2^17, 1 MB of x), rows into `parts` workgroup parts (0 = 1024)
- * whose entries are reordered slab by slab, so that resident workgroups sweep
- * the slabs in step and gather x from L2.  No reference counterpart: the
+ * (<= 2^18; 0 = 2^17, 1 MB of x), rows into `parts` workgroup parts (0 = 512)
+ * whose entries are reordered slab by slab, and by column inside each
+ * (part, slab) segment, so that resident workgroups sweep the slabs in step
+ * and gather x from L2, a wave's lanes from nearby columns.  compute's
+ * `unroll` is the entries per thread (0 = 4).  No reference counterpart: the
  * reference's nearest analogue is packing a tensor into another format
  * before compute (src/tensor.cpp pack()).  Built on the device from A's
  * arrays; A's values are copied, so a plan reflects A at creation time.
@@ -200,7 +202,8 @@ void taco_hip_spmv_plan_destroy(taco_hip_spmv_plan* plan);
 int taco_hip_spmv_plan_info(const taco_hip_spmv_plan* plan, int64_t* info);
 /* Launch form of the main kernel: workgroup size 256 (default) or 512, and
  * whether runs of one row are summed across the wave before the LDS
- * atomics (1, default) or every entry adds its own product (0). */
+ * atomics (1) or each thread adds its runs of one row itself (0, default;
+ * with column-ordered segments nearly every entry is its own run). */
 int taco_hip_spmv_plan_configure(taco_hip_spmv_plan* plan, int32_t threads,
                                  int32_t wave_reduce);
 /* Kernel timing: HIP events bracket the main kernel (k_slab_spmv) of the

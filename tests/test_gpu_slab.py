@@ -11,7 +11,7 @@ from tests import gpu_util as G
 pytestmark = pytest.mark.gpu
 
 
-def run(A, x, parts=0, slab=0, unroll=0, threads=0, wave_reduce=1):
+def run(A, x, parts=0, slab=0, unroll=0, threads=0, wave_reduce=0):
     a = G.csr_desc(A)
     plan = _lib.SpmvPlan(a, parts, slab)
     plan.configure(threads, wave_reduce)
@@ -70,3 +70,19 @@ def test_slab_plan_rejects_f32():
     A = W.csr_fixed(100, 100, 3, 1)
     with pytest.raises(_lib.TacoHipError, match="f64"):
         _lib.SpmvPlan(G.csr_desc(A, "f32"))
+
+
+def test_slab_plan_library_defaults():
+    """The plan as the library builds it by default: 512 parts, segments
+    ordered by column, 4 entries per thread, per-thread run atomics."""
+    A = W.csr_powerlaw(200000, 180000, seed=48, kmax=50000)
+    x = W.uniform(A.n, 7, W.STREAM_X)
+    a = G.csr_desc(A)
+    plan = _lib.SpmvPlan(a)
+    assert plan.info()["parts"] == 512
+    y = torch.full((A.m,), float("nan"), dtype=torch.float64, device="cuda")
+    plan.compute(D.dense(y), D.dense(G.dev(x)))
+    torch.cuda.synchronize()
+    ye, cond = oracle.exact_spmv(A, x)
+    rep = oracle.error_report(y.cpu().numpy(), ye, cond, 1e-12)
+    assert rep["ok"], rep
