@@ -94,14 +94,25 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(
 // ---- one-sort assemble ----
 
 // cand[i] = candidates of row i (saturated at INT_MAX; the prefix sum then
-// reports the overflow).
+// reports the overflow).  kCandLanes lanes per row, each striding the row's
+// A entries, so a wave covers 64 / kCandLanes rows and a row's loads are in
+// flight together.
+constexpr int kCandLanes = 8;
 __global__ __launch_bounds__(256) void k_spgemm_row_cands(
     int32_t m, const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
     const int32_t* __restrict__ bpos, int32_t* __restrict__ cand) {
-  const int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  if (i >= m) return;
-  const int64_t n = taco_hip_row_candidates(apos, acrd, bpos, (int32_t)i);
-  if (__lane_id() == 0) cand[i] = n > INT_MAX ? INT_MAX : (int32_t)n;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / kCandLanes;
+  const int sub = (int)(t % kCandLanes);
+  int64_t n = 0;
+  if (i < m)
+    for (int32_t p = apos[i] + sub; p < apos[i + 1]; p += kCandLanes) {
+      const int32_t j = acrd[p];
+      n += bpos[j + 1] - bpos[j];
+    }
+#pragma unroll
+  for (int off = kCandLanes / 2; off > 0; off >>= 1) n += __shfl_xor(n, off, kCandLanes);
+  if (i < m && sub == 0) cand[i] = n > INT_MAX ? INT_MAX : (int32_t)n;
 }
 
 // Row i's sorted distinct columns into dst[0..count).  Returns the count,
@@ -219,17 +230,31 @@ __global__ __launch_bounds__(256) void k_spgemm_long_dedup(
   if (lane == 0) counts[i] = base;
 }
 
-// crd: row i's sorted columns copied from its scratch region.
+// crd: each row's sorted columns copied from its scratch region.  A
+// workgroup takes kCompactRows consecutive rows, whose output range in crd
+// is contiguous: their pos and region starts go to LDS, and the threads
+// stride the range, each finding its row by binary search in LDS, so the
+// stores are coalesced whatever the row lengths.
+constexpr int kCompactRows = 256;
 __global__ __launch_bounds__(256) void k_spgemm_compact(
     int32_t m, const int32_t* __restrict__ candoff, const int32_t* __restrict__ cpos,
     const int32_t* __restrict__ scratch, int32_t* __restrict__ ccrd) {
-  const int64_t stride = (int64_t)gridDim.x * kWavesPerBlock;
-  const int lane = (int)__lane_id();
-  for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; i < m;
-       i += stride) {
-    const int32_t c0 = cpos[i], n = cpos[i + 1] - c0;
-    const int32_t* src = scratch + candoff[i];
-    for (int32_t q = lane; q < n; q += kWave) __builtin_nontemporal_store(src[q], ccrd + c0 + q);
+  __shared__ int32_t s_pos[kCompactRows + 1];
+  __shared__ int32_t s_src[kCompactRows];
+  const int64_t r0 = (int64_t)blockIdx.x * kCompactRows;
+  const int32_t nr = (int32_t)(m - r0 < kCompactRows ? m - r0 : kCompactRows);
+  for (int q = threadIdx.x; q <= nr; q += blockDim.x) s_pos[q] = cpos[r0 + q];
+  for (int q = threadIdx.x; q < nr; q += blockDim.x) s_src[q] = candoff[r0 + q];
+  __syncthreads();
+  const int32_t e0 = s_pos[0], e1 = s_pos[nr];
+  for (int32_t e = e0 + (int32_t)threadIdx.x; e < e1; e += (int32_t)blockDim.x) {
+    // the row r of output entry e: largest r with s_pos[r] <= e
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_pos[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    __builtin_nontemporal_store(scratch[s_src[lo] + (e - s_pos[lo])], ccrd + e);
   }
 }
 
@@ -509,7 +534,8 @@ int assemble_one_sort(const CsrView& a, const CsrView& b, taco_tensor_t* C, hipS
       break;
     }
     if (m > 0)
-      k_spgemm_row_cands<<<grid, 256, 0, st>>>(m, a.pos, a.crd, b.pos, counts);
+      k_spgemm_row_cands<<<(unsigned)(((int64_t)m * kCandLanes + 255) / 256), 256, 0, st>>>(
+          m, a.pos, a.crd, b.pos, counts);
     if ((rc = device_counts_to_pos(m, counts, candoff, status + 2, st))) break;
     size_t cap_bytes = workspace_size(kSlotScratch);
     int32_t* scratch = (int32_t*)workspace(cap_bytes ? cap_bytes : 4096, kSlotScratch);
@@ -585,8 +611,8 @@ int assemble_one_sort(const CsrView& a, const CsrView& b, taco_tensor_t* C, hipS
       break;
     }
     if (m > 0 && nnz > 0)
-      k_spgemm_compact<<<(unsigned)taco_hip_row_grid(m), 256, 0, st>>>(m, candoff, cpos,
-                                                                        scratch, ccrd);
+      k_spgemm_compact<<<(unsigned)(((int64_t)m + kCompactRows - 1) / kCompactRows), 256, 0,
+                         st>>>(m, candoff, cpos, scratch, ccrd);
     if ((rc = finish("taco_hip_spgemm_assemble"))) {
       taco_hip_unified_free(ccrd);
       taco_hip_unified_free(cvals);
