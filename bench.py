@@ -333,7 +333,7 @@ def bench_spmv(args, A, x, name, workload_desc, spmm=None):
                "kind": kind, "ms": cms, "median_ms": float(np.median(cpu_ts)),
                "sample": f"whole workload, {len(cpu_ts)} timed compute() calls "
                          f"(10% trimmed mean), JIT excluded, inputs in host DRAM"}
-    traffic = profiled_traffic("k_slab_spmv", name)
+    traffic = profiled_traffic("k_slab_spmv_il", name)
     kernel_gbs = nbytes / (kernel_ms * 1e-3) / 1e9
     spmm_legs = None
     if spmm is not None:
@@ -353,8 +353,9 @@ def bench_spmv(args, A, x, name, workload_desc, spmm=None):
         "dtype": "f64", "data": "synthetic (seed-generated)",
         "config": {"workload": name, "desc": workload_desc, "m": A.m, "n": A.n,
                    "nnz": A.nnz, "schedule": f"slab-tiled plan parts={plan_info['parts']} "
-                                             f"slab_width={plan_info['slab_width']} unroll={args.unroll or 4} "
-                                             f"threads=256, segments ordered by column",
+                                             f"slab_width={plan_info['slab_width']} unroll={args.unroll or 8} "
+                                             f"threads=256, interleaved entries, segments "
+                                             f"ordered by column",
                    "parallelism": "single GPU"},
         "pct_of_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS,
         "gflops": 2.0 * A.nnz / (ms * 1e-3) / 1e9,
@@ -368,7 +369,7 @@ def bench_spmv(args, A, x, name, workload_desc, spmm=None):
         "cold": cold,
         "roofline": {"bound": "hbm", "achieved": kernel_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kernel_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_slab_spmv", "kernel_us": kernel_ms * 1e3,
+                     "kernel": "k_slab_spmv_il", "kernel_us": kernel_ms * 1e3,
                      "kernel_launches_timed": kernel_launches,
                      "algorithmic_bytes_per_launch": nbytes},
         "error_vs_fp64_oracle": errors,
@@ -436,8 +437,9 @@ def spmm_leg(args, A, dtype, pos, crd, B, exact, exact_bf16, timer, cpu=None):
         from taco_amd import selector as S
         c, s_l, pred_ms = S.Selector().choose(a, 1, K)
         ms_l, _ = timer.run(step(s_l), max(5, args.steps // 2), 2)
-        errors["learned_" + S.CLASS_NAMES[c]] = fp64_error(result(), exact, tol)
-        learned = {"class": S.CLASS_NAMES[c], "ms": ms_l, "predicted_ms": pred_ms,
+        errors["learned_" + S.class_name(c)] = fp64_error(result(), exact, tol)
+        learned = {"class": S.class_name(c), "ms": ms_l,
+                   "predicted_ms": None if pred_ms != pred_ms else pred_ms,
                    "gflops": 2.0 * A.nnz * K / (ms_l * 1e-3) / 1e9}
     except (OSError, ValueError) as e:  # no trained weights in this tree
         learned = {"error": str(e)}

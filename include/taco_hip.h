@@ -186,7 +186,9 @@ int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,
  * whose entries are reordered slab by slab, and by column inside each
  * (part, slab) segment, so that resident workgroups sweep the slabs in step
  * and gather x from L2, a wave's lanes from nearby columns.  compute's
- * `unroll` is the entries per thread (0 = 4).  No reference counterpart: the
+ * `unroll` is the entries per thread (0 = the kernel form's default: 8 for
+ * the interleaved form, 4 for the others; 1, 2, 4, 8, and 16 for the
+ * interleaved form at 256 threads).  No reference counterpart: the
  * reference's nearest analogue is packing a tensor into another format
  * before compute (src/tensor.cpp pack()).  Built on the device from A's
  * arrays; A's values are copied, so a plan reflects A at creation time.
@@ -201,11 +203,14 @@ void taco_hip_spmv_plan_destroy(taco_hip_spmv_plan* plan);
 /* info[0..4] = parts, slabs, slab width, row weight, max rows in a part. */
 int taco_hip_spmv_plan_info(const taco_hip_spmv_plan* plan, int64_t* info);
 /* Launch form of the main kernel: workgroup size 256 (default) or 512, and
- * whether runs of one row are summed across the wave before the LDS
- * atomics (1) or each thread adds its runs of one row itself (0, default;
- * with column-ordered segments nearly every entry is its own run). */
+ * the kernel form: 2 (default) thread t of the workgroup takes entries t,
+ * t + T, ... of a segment, so a wave's load covers 64 consecutive
+ * column-ordered entries (8 entries per thread by default); 0 each thread
+ * takes consecutive entries and adds its runs of one row itself (4 by
+ * default); 1 runs of one row are summed across the wave before the LDS
+ * atomics (4 by default). */
 int taco_hip_spmv_plan_configure(taco_hip_spmv_plan* plan, int32_t threads,
-                                 int32_t wave_reduce);
+                                 int32_t form);
 /* Kernel timing: HIP events bracket the main kernel (k_slab_spmv) of the
  * next `launches` compute calls on their stream (0 stops and releases).
  * plan_kernel_time waits for the last recorded call and returns the summed

@@ -400,6 +400,88 @@ __device__ inline K taco_hip_wave_sort(K v) {
   return v;
 }
 
+/* The value of lane (lane ^ J), J a power of two below 64, by register
+ * moves only: DPP quad permutes for 1 and 2, two row shifts for 4, a row
+ * rotation by 8 for 8, and gfx950's lane-half swaps for 16 and 32 (no LDS
+ * crossbar, unlike a general shuffle).  Every lane of the wave must be
+ * active: take the result into a variable before any branch or ?: on it. */
+template <int J>
+__device__ inline uint32_t taco_hip_lane_xor_u32(uint32_t v) {
+  const int lane = (int)__lane_id();
+  const int s = (int)v;
+  if constexpr (J == 1) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, s, 0xB1, 0xF, 0xF, false);  /* quad_perm 1,0,3,2 */
+  } else if constexpr (J == 2) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, s, 0x4E, 0xF, 0xF, false);  /* quad_perm 2,3,0,1 */
+  } else if constexpr (J == 4) {
+    const int from_above = __builtin_amdgcn_update_dpp(0, s, 0x104, 0xF, 0xF, false);  /* row_shl:4 */
+    const int from_below = __builtin_amdgcn_update_dpp(0, s, 0x114, 0xF, 0xF, false);  /* row_shr:4 */
+    return (uint32_t)((lane & 4) ? from_below : from_above);
+  } else if constexpr (J == 8) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, s, 0x128, 0xF, 0xF, false);  /* row_ror:8 */
+  } else if constexpr (J == 16) {
+    /* Swaps rows 1 and 3 of the first operand with rows 0 and 2 of the
+     * second: lanes of odd rows find their partner in the first result,
+     * lanes of even rows in the second. */
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & 16) ? r[0] : r[1];
+  } else {
+    static_assert(J == 32, "lane xor distance");
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane & 32) ? r[0] : r[1];
+  }
+}
+
+/* Inclusive prefix sum across the 64-lane wave by DPP moves: row shifts by
+ * 1, 2, 4 and 8 within each 16-lane row, then the last lane of row 0 added
+ * to row 1 and of row 2 to row 3 (row_bcast:15), and the last lane of row 1
+ * to rows 2 and 3 (row_bcast:31).  Every lane of the wave must be active. */
+__device__ inline int32_t taco_hip_wave_scan_add_i32(int32_t v) {
+  int32_t u;
+  u = __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false); /* row_shr:1 */
+  v += u;
+  u = __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false); /* row_shr:2 */
+  v += u;
+  u = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false); /* row_shr:4 */
+  v += u;
+  u = __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false); /* row_shr:8 */
+  v += u;
+  u = __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false); /* row_bcast:15 */
+  v += u;
+  u = __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false); /* row_bcast:31 */
+  v += u;
+  return v;
+}
+
+template <int K, int J>
+__device__ inline uint32_t taco_hip_bitonic_step_u32(uint32_t v) {
+  const int lane = (int)__lane_id();
+  const uint32_t p = taco_hip_lane_xor_u32<J>(v);
+  const bool keep_min = ((lane & K) == 0) == ((lane & J) == 0);
+  return keep_min ? (p < v ? p : v) : (p > v ? p : v);
+}
+
+template <int K>
+__device__ inline uint32_t taco_hip_bitonic_merge_u32(uint32_t v) {
+  if constexpr (K >= 64) v = taco_hip_bitonic_step_u32<K, 32>(v);
+  if constexpr (K >= 32) v = taco_hip_bitonic_step_u32<K, 16>(v);
+  if constexpr (K >= 16) v = taco_hip_bitonic_step_u32<K, 8>(v);
+  if constexpr (K >= 8) v = taco_hip_bitonic_step_u32<K, 4>(v);
+  if constexpr (K >= 4) v = taco_hip_bitonic_step_u32<K, 2>(v);
+  return taco_hip_bitonic_step_u32<K, 1>(v);
+}
+
+/* taco_hip_wave_sort for 32-bit unsigned keys, every exchange a register
+ * move (taco_hip_lane_xor_u32). */
+__device__ inline uint32_t taco_hip_wave_sort_u32(uint32_t v) {
+  v = taco_hip_bitonic_merge_u32<2>(v);
+  v = taco_hip_bitonic_merge_u32<4>(v);
+  v = taco_hip_bitonic_merge_u32<8>(v);
+  v = taco_hip_bitonic_merge_u32<16>(v);
+  v = taco_hip_bitonic_merge_u32<32>(v);
+  return taco_hip_bitonic_merge_u32<64>(v);
+}
+
 /* Workgroups for a grid-strided row loop over m rows, 4 rows (waves) per
  * workgroup: at most 4 workgroups per CU, which every row kernel here keeps
  * resident at once. */
@@ -487,13 +569,8 @@ __device__ inline int64_t taco_hip_row_lanes(
   /* B row lengths clamped to 65: the prefix sums stay in int32 and a total
    * above 64 still reads as above 64. */
   const int32_t lc = len < 65 ? len : 65;
-  int32_t inc = lc;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int32_t u = __shfl_up(inc, off, 64);
-    if (lane >= off) inc += u;
-  }
-  const int32_t total = __shfl(inc, 63, 64);
+  const int32_t inc = taco_hip_wave_scan_add_i32(lc);
+  const int32_t total = __builtin_amdgcn_readlane(inc, 63);
   if (total > 64) {
     int64_t n = len;
 #pragma unroll
@@ -503,7 +580,7 @@ __device__ inline int64_t taco_hip_row_lanes(
   /* Entry t holding candidate `lane`: the number of entries s < la - 1 whose
    * inclusive sum is <= lane. */
   int t = 0;
-  for (int s2 = 0; s2 + 1 < la; ++s2) t += __shfl(inc, s2, 64) <= lane ? 1 : 0;
+  for (int s2 = 0; s2 + 1 < la; ++s2) t += __builtin_amdgcn_readlane(inc, s2) <= lane ? 1 : 0;
   const int32_t excl = __shfl(inc - lc, t, 64);
   const int32_t bt = __shfl(b0, t, 64);
   if (lane < total) {
@@ -594,7 +671,7 @@ __device__ inline int32_t taco_hip_spgemm_row_symbolic(
       taco_hip_row_candidate(apos, acrd, bpos, bcrd, i, lane, &col, &jA, &kB);
   }
   if (n <= 64) {
-    const int32_t s = taco_hip_wave_sort(col);
+    const int32_t s = (int32_t)taco_hip_wave_sort_u32((uint32_t)col);
     const int32_t prev = __shfl_up(s, 1, 64);
     const bool head = s != INT_MAX && (lane == 0 || prev != s);
     const uint64_t heads = __ballot(head);
@@ -732,6 +809,68 @@ __device__ inline void taco_hip_spgemm_row_numeric_global(
 }
 
 /* Numeric row on an assembled (sorted) pattern. */
+/* Numeric row of at most 64 candidates, held in lanes (taco_hip_row_lanes):
+ * products sorted by (column, encounter index) and summed per column in
+ * encounter order; the row's values go to cvals[c0 .. c0 + nc). */
+__device__ inline void taco_hip_spgemm_row_numeric_short(
+    int64_t n, int32_t col, int32_t jA, int32_t kB, const double* avals,
+    const double* bvals, int32_t c0, int32_t nc, double* cvals) {
+  const int lane = (int)__lane_id();
+  double prod = 0;
+  if (lane < n) prod = avals[jA] * bvals[kB];
+  /* Key = column, then encounter index (which also names the lane): in
+   * 32 bits when every column is below 2^26, else in 64. */
+  int32_t scol, src;
+  if (__ballot(col != INT_MAX && col >= (1 << 26)) == 0) {
+    uint32_t key = col == INT_MAX ? 0xFFFFFFFFu : ((uint32_t)col << 6) | (uint32_t)lane;
+    key = taco_hip_wave_sort_u32(key);
+    scol = key == 0xFFFFFFFFu ? INT_MAX : (int32_t)(key >> 6);
+    src = (int32_t)(key & 63);
+  } else {
+    uint64_t key = ((uint64_t)(uint32_t)col << 32) | (uint32_t)lane;
+    key = taco_hip_wave_sort(key);
+    scol = (int32_t)(key >> 32);
+    src = (int32_t)(key & 63);
+  }
+  const double v = __shfl(prod, src, 64);
+  const int32_t prev = __shfl_up(scol, 1, 64);
+  const bool valid = scol != INT_MAX;
+  const bool head = valid && (lane == 0 || prev != scol);
+  const uint64_t heads = __ballot(head);
+  const int nvalid = __popcll(__ballot(valid));
+  int run = 0;
+  if (head) {
+    const uint64_t above = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1);
+    const int next = above ? __ffsll((unsigned long long)above) - 1 : nvalid;
+    run = next - lane;
+  }
+  /* Wave maximum of the run lengths.  Each lane move is taken in full-wave
+   * control flow, before the comparison: a lane move under a branch would
+   * read lanes the branch has switched off. */
+  uint32_t maxrun = (uint32_t)run, other;
+  other = taco_hip_lane_xor_u32<32>(maxrun);
+  maxrun = maxrun > other ? maxrun : other;
+  other = taco_hip_lane_xor_u32<16>(maxrun);
+  maxrun = maxrun > other ? maxrun : other;
+  other = taco_hip_lane_xor_u32<8>(maxrun);
+  maxrun = maxrun > other ? maxrun : other;
+  other = taco_hip_lane_xor_u32<4>(maxrun);
+  maxrun = maxrun > other ? maxrun : other;
+  other = taco_hip_lane_xor_u32<2>(maxrun);
+  maxrun = maxrun > other ? maxrun : other;
+  other = taco_hip_lane_xor_u32<1>(maxrun);
+  maxrun = maxrun > other ? maxrun : other;
+  double s = v;
+  for (int q = 1; q < (int)maxrun; ++q) {
+    const double u = __shfl(v, (lane + q) & 63, 64);
+    if (head && q < run) s = s + u;
+  }
+  if (head) {
+    const int rank = __popcll(heads & ((1ull << lane) - 1));
+    if (rank < nc) __builtin_nontemporal_store(s, cvals + c0 + rank); /* written once */
+  }
+}
+
 __device__ inline void taco_hip_spgemm_row_numeric(
     const int32_t* apos, const int32_t* acrd, const double* avals,
     const int32_t* bpos, const int32_t* bcrd, const double* bvals, int32_t i,
@@ -747,39 +886,7 @@ __device__ inline void taco_hip_spgemm_row_numeric(
       taco_hip_row_candidate(apos, acrd, bpos, bcrd, i, lane, &col, &jA, &kB);
   }
   if (n <= 64) {
-    double prod = 0;
-    if (lane < n) prod = avals[jA] * bvals[kB];
-    /* Key = column, then encounter index (which also names the lane). */
-    uint64_t key = ((uint64_t)(uint32_t)col << 32) | (uint32_t)lane;
-    key = taco_hip_wave_sort(key);
-    const int32_t scol = (int32_t)(key >> 32);
-    const double v = __shfl(prod, (int)(key & 63), 64);
-    const int32_t prev = __shfl_up(scol, 1, 64);
-    const bool valid = scol != INT_MAX;
-    const bool head = valid && (lane == 0 || prev != scol);
-    const uint64_t heads = __ballot(head);
-    const int nvalid = __popcll(__ballot(valid));
-    int run = 0;
-    if (head) {
-      const uint64_t above = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1);
-      const int next = above ? __ffsll((unsigned long long)above) - 1 : nvalid;
-      run = next - lane;
-    }
-    int maxrun = run;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const int o = __shfl_xor(maxrun, off, 64);
-      maxrun = o > maxrun ? o : maxrun;
-    }
-    double s = v;
-    for (int q = 1; q < maxrun; ++q) {
-      const double u = __shfl(v, (lane + q) & 63, 64);
-      if (head && q < run) s = s + u;
-    }
-    if (head) {
-      const int rank = __popcll(heads & ((1ull << lane) - 1));
-      if (rank < nc) __builtin_nontemporal_store(s, cvals + c0 + rank); /* written once */
-    }
+    taco_hip_spgemm_row_numeric_short(n, col, jA, kB, avals, bvals, c0, nc, cvals);
     return;
   }
   if (nc > TACO_HIP_ROW_CAP) {

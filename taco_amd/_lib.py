@@ -252,8 +252,10 @@ class SpmvPlan:
         return dict(zip(("parts", "slabs", "slab_width", "row_weight",
                          "max_local_rows"), list(out)))
 
-    def configure(self, threads=0, wave_reduce=1):
-        check(lib().taco_hip_spmv_plan_configure(self._p, threads, wave_reduce), "plan configure")
+    def configure(self, threads=0, form=2):
+        """Workgroup size (256 or 512) and kernel form: 2 interleaved entries
+        (the default), 0 consecutive entries per thread, 1 wave-summed runs."""
+        check(lib().taco_hip_spmv_plan_configure(self._p, threads, form), "plan configure")
 
     def time_kernel(self, launches):
         """Bracket the main kernel of the next `launches` computes with events."""

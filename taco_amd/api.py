@@ -397,6 +397,8 @@ class _Learned:
         if _learned["selector"] is None:
             _learned["selector"] = selector.Selector()
         sel = _learned["selector"]
+        if selector.mfma_rule(a_desc, self.op, K):
+            return selector.RULE_MFMA, selector.class_schedule(selector.RULE_MFMA, K)
         z = sel.scores(selector.features(a_desc, self.op, K))[0]
         lo, hi = selector.class_range(self.op)
         for c in lo + np.argsort(-z[lo:hi], kind="stable"):
@@ -436,7 +438,7 @@ class _Learned:
         """Name of the class picked for the operand (None before compute)."""
         from . import selector
         ch = self.A._schedule_choice.get((self.op, K))
-        return None if ch is None else selector.CLASS_NAMES[ch[0]]
+        return None if ch is None else selector.class_name(ch[0])
 
 
 # --------------------------------------------------------------- tensors --

@@ -91,6 +91,51 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(
 }
 
 
+// Numeric pass over the rows of at most 64 candidates, whose products stay
+// in lanes: this kernel uses no LDS, so twice as many waves stay resident as
+// with k_spgemm_numeric (whose long-row accumulators take 32 KB of LDS per
+// workgroup), and more rows' dependent loads are in flight.  Longer rows are
+// listed (long_rows, *nlong) for k_spgemm_numeric_listed.
+__global__ __launch_bounds__(256) void k_spgemm_numeric_short(
+    int32_t m, const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
+    const double* __restrict__ avals, const int32_t* __restrict__ bpos,
+    const int32_t* __restrict__ bcrd, const double* __restrict__ bvals,
+    const int32_t* __restrict__ cpos, double* __restrict__ cvals,
+    int32_t* __restrict__ long_rows, int32_t* __restrict__ nlong) {
+  const int64_t stride = (int64_t)gridDim.x * kWavesPerBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; i < m;
+       i += stride) {
+    const int32_t c0 = cpos[i], nc = cpos[i + 1] - c0;
+    int32_t col = INT_MAX, jA = 0, kB = 0;
+    const int64_t n = taco_hip_row_lanes(apos, acrd, bpos, bcrd, (int32_t)i, &col, &jA, &kB);
+    if (n < 0 || n > 64) {
+      if (__lane_id() == 0) long_rows[atomicAdd(nlong, 1)] = (int32_t)i;
+      continue;
+    }
+    taco_hip_spgemm_row_numeric_short(n, col, jA, kB, avals, bvals, c0, nc, cvals);
+  }
+}
+
+// Numeric pass over the rows listed by k_spgemm_numeric_short.
+__global__ __launch_bounds__(256) void k_spgemm_numeric_listed(
+    const int32_t* __restrict__ long_rows, const int32_t* __restrict__ nlong,
+    const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
+    const double* __restrict__ avals, const int32_t* __restrict__ bpos,
+    const int32_t* __restrict__ bcrd, const double* __restrict__ bvals,
+    const int32_t* __restrict__ cpos, const int32_t* __restrict__ ccrd,
+    double* __restrict__ cvals) {
+  const int32_t count = *nlong;
+  const int64_t stride = (int64_t)gridDim.x * kWavesPerBlock;
+  for (int64_t r = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; r < count;
+       r += stride)
+    taco_hip_spgemm_row_numeric(apos, acrd, avals, bpos, bcrd, bvals, long_rows[r], cpos,
+                                ccrd, cvals);
+}
+
+__global__ void k_spgemm_zero_count(int32_t* __restrict__ count) {
+  if (threadIdx.x == 0) *count = 0;
+}
+
 // ---- one-sort assemble ----
 
 // cand[i] = candidates of row i (saturated at INT_MAX; the prefix sum then
@@ -131,7 +176,7 @@ __device__ inline int32_t row_sorted_set(const int32_t* apos, const int32_t* acr
       taco_hip_row_candidate(apos, acrd, bpos, bcrd, i, lane, &col, &jA, &kB);
   }
   if (n <= 64) {
-    const int32_t s = taco_hip_wave_sort(col);
+    const int32_t s = (int32_t)taco_hip_wave_sort_u32((uint32_t)col);
     const int32_t prev = __shfl_up(s, 1, 64);
     const bool head = s != INT_MAX && (lane == 0 || prev != s);
     const uint64_t heads = __ballot(head);
@@ -456,9 +501,22 @@ int assemble_two_pass(const CsrView& a, const CsrView& b, taco_tensor_t* C,
   return rc;
 }
 
-// Workspace slots of the one-sort assemble.
+// Workspace slots of the one-sort assemble, and of compute's row list.
 constexpr int kSlotCandOff = 8, kSlotScratch = 9, kSlotLongMeta = 10, kSlotLongKeys = 11,
-              kSlotSortTemp = 12;
+              kSlotSortTemp = 12, kSlotComputeList = 13;
+// Workgroups per CU of k_spgemm_numeric_short's resident grid.
+constexpr int kShortBlocksPerCu = 8;
+
+int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+    cus = n > 0 ? n : 256;
+  }
+  return cus;
+}
 
 // Rows listed by the symbolic pass (more than kHashCap distinct columns):
 // candidates written out, sorted per row in global memory, deduplicated into
@@ -644,9 +702,20 @@ extern "C" int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,
   if (int rc = view_csr(C, "C", &c)) return rc;
   if (a.m == 0 || c.nnz == 0) return finish("taco_hip_spgemm_compute");
   hipStream_t st = current_stream();
-  const unsigned grid = (unsigned)taco_hip_row_grid(a.m);
-  k_spgemm_numeric<<<grid, 256, 0, st>>>(
-      a.m, a.pos, a.crd, (const double*)a.vals, b.pos, b.crd,
+  // Workspace: the count of listed rows, then the list.
+  char* ws = (char*)workspace(256 + sizeof(int32_t) * (size_t)a.m, kSlotComputeList);
+  if (!ws) return 1;
+  int32_t* nlong = (int32_t*)ws;
+  int32_t* long_rows = (int32_t*)(ws + 256);
+  k_spgemm_zero_count<<<1, 64, 0, st>>>(nlong);
+  const int64_t short_grid = (a.m + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t resident = (int64_t)kShortBlocksPerCu * device_cus();
+  k_spgemm_numeric_short<<<(unsigned)(short_grid < resident ? short_grid : resident), 256, 0,
+                           st>>>(a.m, a.pos, a.crd, (const double*)a.vals, b.pos, b.crd,
+                                 (const double*)b.vals, c.pos, (double*)c.vals, long_rows,
+                                 nlong);
+  k_spgemm_numeric_listed<<<(unsigned)taco_hip_row_grid(a.m), 256, 0, st>>>(
+      long_rows, nlong, a.pos, a.crd, (const double*)a.vals, b.pos, b.crd,
       (const double*)b.vals, c.pos, c.crd, (double*)c.vals);
   return finish("taco_hip_spgemm_compute");
 }

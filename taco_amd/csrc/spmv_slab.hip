@@ -31,11 +31,17 @@
 // sweep over 384..1024 parts, 1..8 entries per thread, 256/512 threads and
 // 1/2 MB slabs).
 //
-// Two forms of the main kernel accumulate the row partials in LDS:
-//   k_slab_spmv (default): a thread takes U consecutive entries (vector
-//     loads), sums the runs of one row among them and adds each run with an
-//     LDS atomic — with column-ordered segments nearly every entry is its
-//     own run;
+// Three forms of the main kernel accumulate the row partials in LDS:
+//   k_slab_spmv_il (default, 8 entries per thread): the workgroup takes
+//     8·T consecutive entries at a time and thread t takes entries t, t + T,
+//     ..., so one load instruction of a wave covers 64 consecutive entries
+//     of the column-ordered segment and lanes whose columns share a line of
+//     x are served by one L2 request; each product is added to its row with
+//     an LDS atomic (195 us on the power-law matrix);
+//   k_slab_spmv: a thread takes U consecutive entries (vector loads), sums
+//     the runs of one row among them and adds each run with an LDS atomic —
+//     with column-ordered segments nearly every entry is its own run (205 us
+//     with 4 entries per thread);
 //   k_slab_spmv_wave: a wave takes blocks of 64 consecutive entries; a
 //     segmented wave sum by row (DPP row shifts and broadcasts) leaves one
 //     total per row run, added by one lane.  It pays off when segments keep
@@ -64,7 +70,8 @@ namespace {
 constexpr int kColBits = 18;
 constexpr uint32_t kColMask = (1u << kColBits) - 1;
 constexpr int kPlanThreads = 256;
-constexpr int kDefaultUnroll = 4;
+constexpr int kDefaultUnroll = 4;     // k_slab_spmv, k_slab_spmv_wave
+constexpr int kDefaultUnrollIl = 8;   // k_slab_spmv_il
 constexpr int kDefaultParts = 512;
 // 2^17 columns = 1 MB of f64 x per slab (a quarter of an XCD's L2).
 constexpr int kDefaultSlabWidth = 1 << 17;
@@ -320,6 +327,63 @@ __global__ __launch_bounds__(T) void k_slab_spmv(
   }
 }
 
+// One workgroup per part.  The workgroup takes U·T consecutive entries of
+// the current slab segment at a time and thread t takes entries t, t + T,
+// ..., so one load instruction of a wave covers 64 consecutive entries: in a
+// column-ordered segment, lanes whose columns share a cache line of x are
+// served by one L2 request.  Each entry's product goes to its row's LDS
+// partial with one atomic.
+template <int U, int T>
+__global__ __launch_bounds__(T) void k_slab_spmv_il(
+    int32_t m, int32_t slabs, int32_t slab_width,
+    const int32_t* __restrict__ part_row, const int32_t* __restrict__ off,
+    const int32_t* __restrict__ seg_end,
+    const uint32_t* __restrict__ ent, const double* __restrict__ val2,
+    const double* __restrict__ x, double* __restrict__ y,
+    int32_t* __restrict__ carry_row, double* __restrict__ carry_val) {
+  extern __shared__ double s_part[];
+  const int32_t p = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int32_t r0 = part_row[p], r1 = part_row[p + 1];
+  const int32_t nrows = r1 - r0 + 1;  // local rows, <= kMaxLocalRows by plan
+  for (int32_t q = tid; q < nrows; q += T) s_part[q] = 0;
+  __syncthreads();
+  const int32_t* o = off + (int64_t)p * (slabs + 1);
+  const int32_t* se = seg_end + (int64_t)p * slabs;
+  const int32_t s_first = first_slab(p, slabs);
+  for (int32_t s0 = 0; s0 < slabs; ++s0) {
+    const int32_t s = s0 + s_first < slabs ? s0 + s_first : s0 + s_first - slabs;
+    const int32_t e0 = o[s], e1 = se[s];
+    const double* xs = x + (int64_t)s * slab_width;
+    for (int32_t b = e0; b < e1; b += U * T) {
+      // Threads past the segment end re-read its last entry (a valid
+      // address) and add nothing, so every thread issues the same loads.
+      uint32_t pk[U];
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t e = b + u * T + tid;
+        const int32_t ec = e < e1 ? e : e1 - 1;
+        pk[u] = stream_load(ent + ec);
+        v[u] = stream_load(val2 + ec);
+      }
+      double xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) xv[u] = xs[pk[u] & kColMask];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (b + u * T + tid < e1) atomicAdd(&s_part[pk[u] >> kColBits], v[u] * xv[u]);
+    }
+  }
+  __syncthreads();
+  // Rows r0 .. r1-1 complete here; row r1 is carried.
+  for (int32_t q = tid; q < nrows - 1; q += T) y[r0 + q] = s_part[q];
+  if (tid == 0) {
+    carry_row[p] = r1;
+    carry_val[p] = s_part[nrows - 1];
+  }
+}
+
 // f64 DPP move: lanes whose source is invalid or outside row_mask read 0.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ double dpp_f64(double v) {
@@ -459,7 +523,8 @@ struct taco_hip_spmv_plan {
   double* carry_val = nullptr;    // parts
   int32_t max_local_rows = 0;
   int32_t threads = kPlanThreads;  // workgroup size of the main kernel
-  int32_t wave_reduce = 0;         // k_slab_spmv_wave (1) or k_slab_spmv (0)
+  // Main kernel: k_slab_spmv (0), k_slab_spmv_wave (1), k_slab_spmv_il (2).
+  int32_t form = 2;
   // Optional kernel timing: event pairs around k_slab_spmv of the next
   // `timing_cap` compute calls.
   std::vector<hipEvent_t> timing_ev;
@@ -703,9 +768,10 @@ extern "C" int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* p,
   const double* xp = (const double*)xv.vals;
   double* yp = (double*)yv.vals;
   const size_t lds = sizeof(double) * (size_t)(p->max_local_rows > 0 ? p->max_local_rows : 1);
-  const int u = unroll <= 0 ? kDefaultUnroll : unroll;
+  const int u = unroll > 0 ? unroll : p->form == 2 ? kDefaultUnrollIl : kDefaultUnroll;
   const int th = p->threads;
-  if (u != 1 && u != 2 && u != 4 && u != 8) return fail("spmv plan: unroll %d unsupported", unroll);
+  if (u != 1 && u != 2 && u != 4 && u != 8 && !(u == 16 && p->form == 2 && th == 256))
+    return fail("spmv plan: unroll %d unsupported", unroll);
   const bool timed = p->timing_count < p->timing_cap;
   if (timed) TH_CHECK(hipEventRecord(p->timing_ev[2 * (size_t)p->timing_count], st));
 #define TH_SLAB(U, T)                                                           \
@@ -718,15 +784,24 @@ extern "C" int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* p,
     k_slab_spmv_wave<U, T><<<p->parts, T, lds, st>>>(                           \
         p->m, p->slabs, p->slab_width, p->part_row, p->off, p->seg_end, p->ent, \
         p->val2, xp, yp, p->carry_row, p->carry_val);
-  if (p->wave_reduce) {
+#define TH_SLAB_I(U, T)                                                         \
+  if (u == U && th == T)                                                        \
+    k_slab_spmv_il<U, T><<<p->parts, T, lds, st>>>(                             \
+        p->m, p->slabs, p->slab_width, p->part_row, p->off, p->seg_end, p->ent, \
+        p->val2, xp, yp, p->carry_row, p->carry_val);
+  if (p->form == 1) {
     TH_SLAB_W(1, 256) TH_SLAB_W(2, 256) TH_SLAB_W(4, 256) TH_SLAB_W(8, 256)
     TH_SLAB_W(1, 512) TH_SLAB_W(2, 512) TH_SLAB_W(4, 512) TH_SLAB_W(8, 512)
+  } else if (p->form == 2) {
+    TH_SLAB_I(1, 256) TH_SLAB_I(2, 256) TH_SLAB_I(4, 256) TH_SLAB_I(8, 256) TH_SLAB_I(16, 256)
+    TH_SLAB_I(1, 512) TH_SLAB_I(2, 512) TH_SLAB_I(4, 512) TH_SLAB_I(8, 512)
   } else {
     TH_SLAB(1, 256) TH_SLAB(2, 256) TH_SLAB(4, 256) TH_SLAB(8, 256)
     TH_SLAB(1, 512) TH_SLAB(2, 512) TH_SLAB(4, 512) TH_SLAB(8, 512)
   }
 #undef TH_SLAB
 #undef TH_SLAB_W
+#undef TH_SLAB_I
   TH_LAUNCH_CHECK("k_slab_spmv");
   if (timed) TH_CHECK(hipEventRecord(p->timing_ev[2 * (size_t)p->timing_count++ + 1], st));
   k_slab_fixup<<<(p->parts + 255) / 256, 256, 0, st>>>(p->parts, p->m, p->carry_row,
@@ -738,12 +813,13 @@ extern "C" int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* p,
 extern "C" void taco_hip_spmv_plan_destroy(taco_hip_spmv_plan* p) { plan_free(p); }
 
 extern "C" int taco_hip_spmv_plan_configure(taco_hip_spmv_plan* p, int32_t threads,
-                                           int32_t wave_reduce) {
+                                           int32_t form) {
   TH_REQUIRE(p, "spmv plan: null plan");
   if (threads <= 0) threads = kPlanThreads;
   TH_REQUIRE(threads == 256 || threads == 512, "spmv plan: threads must be 256 or 512");
   p->threads = threads;
-  p->wave_reduce = wave_reduce != 0;
+  TH_REQUIRE(form >= 0 && form <= 2, "spmv plan: kernel form %d unknown", form);
+  p->form = form;
   return 0;
 }
 

@@ -56,16 +56,35 @@ ATOMIC_CLASSES = {c for c, name in enumerate(CLASS_NAMES)
 PLAN_CLASS = SPMV_NAMES.index("slab_plan")
 
 
+# bf16 SpMM goes to the MFMA kernel by rule: the network's classes were
+# timed in f64 and f32, the MFMA kernel exists only for bf16 (K a multiple of
+# 4), and it is the fastest bf16 form measured (0.89 ms on the 2M-row
+# power-law matrix at K = 64, against 2.06 ms for the merge class the
+# network picks there).  Its index lies past the network's classes.
+RULE_MFMA = N_CLASSES
+
+
 def class_range(op):
     return (0, N_SPMV) if op == 0 else (N_SPMV, N_CLASSES)
+
+
+def class_name(c):
+    return "spmm_mfma_bf16 (rule)" if c == RULE_MFMA else CLASS_NAMES[c]
 
 
 def class_schedule(c, K=0):
     """The taco_hip_schedule of class c (SpMM group sizes follow K); None
     for the slab plan."""
+    if c == RULE_MFMA:
+        return autotune._sched(_lib.MFMA)
     if c < N_SPMV:
         return autotune.SPMV_CANDIDATES[c][1]
     return autotune.spmm_candidates(K)[c - N_SPMV][1]
+
+
+def mfma_rule(a_desc, op, K):
+    """True when the bf16 rule applies: SpMM over bf16 values, K % 4 == 0."""
+    return op == 1 and int(a_desc.t.csize) == 16 and K > 0 and K % 4 == 0
 
 
 # ------------------------------------------------------------ features --
@@ -369,7 +388,10 @@ class Selector:
         return out, Z[:, N_CLASSES]
 
     def choose(self, a_desc, op, K=0):
-        """(class, schedule, predicted ms) for one operand."""
+        """(class, schedule, predicted ms) for one operand; bf16 SpMM takes
+        the MFMA rule (no prediction: nan)."""
+        if mfma_rule(a_desc, op, K):
+            return RULE_MFMA, class_schedule(RULE_MFMA, K), float("nan")
         f = features(a_desc, op, K)
         c, lt = self.choose_from_features(f)
         c = int(c[0])

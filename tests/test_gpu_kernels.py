@@ -210,6 +210,22 @@ def spgemm_cases():
     ca[::101] = 1500
     ca[5::257] = 0
     yield "over_row_cap", W.csr_from_counts(2000, 30000, ca, 33), W.csr_fixed(30000, 40000, 4, 34)
+    # Short rows sort (column, encounter) keys in 32 bits while every column
+    # is below 2^26 and in 64 bits otherwise: the first 1,000 rows of A name
+    # B rows whose columns are below 2^20, the rest B rows spread over 2^27
+    # columns.
+    yield "wide_columns", _stack_rows(W.csr_fixed(1000, 1500, 8, 35),
+                                      W.csr_fixed(1000, 1500, 8, 36), 1500), \
+        _stack_rows(W.csr_fixed(1500, 1 << 20, 8, 37), W.csr_fixed(1500, (1 << 27) + 5, 8, 38))
+
+
+def _stack_rows(top, bottom, bottom_col_offset=0):
+    """The rows of `top` above those of `bottom`, whose columns move right by
+    bottom_col_offset; the width is the larger of the two."""
+    pos = np.concatenate([top.pos, bottom.pos[1:] + top.pos[-1]]).astype(np.int32)
+    crd = np.concatenate([top.crd, bottom.crd + bottom_col_offset]).astype(np.int32)
+    n = max(top.n, bottom.n + bottom_col_offset)
+    return W.Csr(top.m + bottom.m, n, pos, crd, np.concatenate([top.vals, bottom.vals]))
 
 
 @pytest.mark.parametrize("name,A,B", list(spgemm_cases()), ids=lambda v: v if isinstance(v, str) else "")
@@ -223,7 +239,7 @@ def test_spgemm(name, A, B):
     assert rep["ok"], rep
     if name == "over_row_cap":
         assert np.diff(pos).max() > 4000  # the global-memory row path ran
-    if name in ("fixed8", "long_a_empty_b"):
+    if name in ("fixed8", "long_a_empty_b", "wide_columns"):
         # Short rows sum duplicates sequentially in taco's encounter order,
         # with separately rounded products: equal to taco's values bit for bit.
         assert np.array_equal(vals, ovals)

@@ -11,10 +11,10 @@ from tests import gpu_util as G
 pytestmark = pytest.mark.gpu
 
 
-def run(A, x, parts=0, slab=0, unroll=0, threads=0, wave_reduce=0):
+def run(A, x, parts=0, slab=0, unroll=0, threads=0, form=2):
     a = G.csr_desc(A)
     plan = _lib.SpmvPlan(a, parts, slab)
-    plan.configure(threads, wave_reduce)
+    plan.configure(threads, form)
     y = torch.full((A.m,), float("nan"), dtype=torch.float64, device="cuda")
     xd = G.dev(x)
     plan.compute(D.dense(y), D.dense(xd), unroll)
@@ -35,19 +35,21 @@ CASES = [
 ]
 
 
+# (entries per thread, threads, kernel form: 0 per-thread runs, 1 wave
+# segmented sums, 2 interleaved entries)
 FORMS = [(1, 256, 1), (2, 256, 1), (4, 512, 1), (8, 256, 1), (1, 256, 0), (4, 256, 0),
-         (8, 512, 0)]
+         (8, 512, 0), (1, 256, 2), (4, 256, 2), (8, 256, 2), (8, 512, 2), (0, 0, 2)]
 
 
 @pytest.mark.parametrize("name,make,parts,slab", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("unroll,threads,wave_reduce", FORMS,
-                         ids=[f"u{u}t{t}w{w}" for u, t, w in FORMS])
-def test_slab_spmv(name, make, parts, slab, unroll, threads, wave_reduce):
+@pytest.mark.parametrize("unroll,threads,form", FORMS,
+                         ids=[f"u{u}t{t}f{f}" for u, t, f in FORMS])
+def test_slab_spmv(name, make, parts, slab, unroll, threads, form):
     A = make()
     print(f"slab spmv {name} m={A.m} nnz={A.nnz} parts={parts} slab={slab} unroll={unroll} "
-          f"threads={threads} wave_reduce={wave_reduce}", flush=True)
+          f"threads={threads} form={form}", flush=True)
     x = W.uniform(A.n, 5, W.STREAM_X)
-    y, info = run(A, x, parts, slab, unroll, threads, wave_reduce)
+    y, info = run(A, x, parts, slab, unroll, threads, form)
     ye, cond = oracle.exact_spmv(A, x)
     rep = oracle.error_report(y, ye, cond, 1e-12)
     assert rep["ok"], (rep, info)
@@ -74,7 +76,7 @@ def test_slab_plan_rejects_f32():
 
 def test_slab_plan_library_defaults():
     """The plan as the library builds it by default: 512 parts, segments
-    ordered by column, 4 entries per thread, per-thread run atomics."""
+    ordered by column, interleaved entries, 8 per thread."""
     A = W.csr_powerlaw(200000, 180000, seed=48, kmax=50000)
     x = W.uniform(A.n, 7, W.STREAM_X)
     a = G.csr_desc(A)

@@ -126,6 +126,27 @@ def test_class_space():
             assert s.group_size == 0 or s.group_size * 4 >= K
 
 
+def test_bf16_spmm_rule():
+    """bf16 SpMM with K a multiple of 4 goes to the MFMA kernel by rule;
+    every other operand is left to the network."""
+    import torch
+    from taco_amd import _lib, descriptors as D
+    pos = torch.zeros(3, dtype=torch.int32)
+    crd = torch.zeros(0, dtype=torch.int32)
+    for dtype, vals in (("bf16", torch.zeros(0, dtype=torch.int16)),
+                        ("f32", torch.zeros(0, dtype=torch.float32)),
+                        ("f64", torch.zeros(0, dtype=torch.float64))):
+        a = D.csr(pos, crd, vals, (2, 2), dtype)
+        assert S.mfma_rule(a, 1, 64) == (dtype == "bf16")
+        assert not S.mfma_rule(a, 0, 0)
+        if dtype == "bf16":
+            assert not S.mfma_rule(a, 1, 30)
+            c, sched, pred = S.Selector.choose(None, a, 1, 64)
+            assert c == S.RULE_MFMA and sched.algo == _lib.MFMA and pred != pred
+            assert S.class_name(c).startswith("spmm_mfma_bf16")
+    assert S.class_name(0) == S.CLASS_NAMES[0]
+
+
 def test_evaluate_totals():
     t = np.full((3, 24), np.inf)
     t[0, :13] = np.arange(13) + 1.0      # default (class 0) = 1, best = 1
