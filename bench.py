@@ -598,7 +598,8 @@ def bench_spgemm(args):
         "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                      "achieved": cmp_bytes / (ms_cmp * 1e-3) / 1e9,
                      "frac": cmp_bytes / (ms_cmp * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_spgemm_numeric",
+                     "traffic": profiled_traffic("k_spgemm_numeric_short", "spgemm_500k_8perrow"),
+                     "kernel": "k_spgemm_numeric_short",
                      "assemble_gbs": asm_bytes / (ms_asm * 1e-3) / 1e9},
         "check_vs_cpu_reference": check,
         "cpu_baseline": cpu,

@@ -809,15 +809,15 @@ __device__ inline void taco_hip_spgemm_row_numeric_global(
 }
 
 /* Numeric row on an assembled (sorted) pattern. */
-/* Numeric row of at most 64 candidates, held in lanes (taco_hip_row_lanes):
- * products sorted by (column, encounter index) and summed per column in
- * encounter order; the row's values go to cvals[c0 .. c0 + nc). */
-__device__ inline void taco_hip_spgemm_row_numeric_short(
-    int64_t n, int32_t col, int32_t jA, int32_t kB, const double* avals,
-    const double* bvals, int32_t c0, int32_t nc, double* cvals) {
+/* Numeric row of at most 64 candidates held in lanes, lane c holding
+ * candidate c (encounter order) with its column and product (lanes c >= n:
+ * column INT_MAX): the products are sorted by (column, encounter index) and
+ * summed per column in encounter order; the row's values go to
+ * cvals[c0 .. c0 + nc).  Every lane of the wave must call it. */
+__device__ inline void taco_hip_spgemm_row_sum_store(int64_t n, int32_t col, double prod,
+                                                     int32_t c0, int32_t nc, double* cvals) {
   const int lane = (int)__lane_id();
-  double prod = 0;
-  if (lane < n) prod = avals[jA] * bvals[kB];
+  (void)n;
   /* Key = column, then encounter index (which also names the lane): in
    * 32 bits when every column is below 2^26, else in 64. */
   int32_t scol, src;
@@ -869,6 +869,15 @@ __device__ inline void taco_hip_spgemm_row_numeric_short(
     const int rank = __popcll(heads & ((1ull << lane) - 1));
     if (rank < nc) __builtin_nontemporal_store(s, cvals + c0 + rank); /* written once */
   }
+}
+
+/* Numeric row of at most 64 candidates, held in lanes (taco_hip_row_lanes). */
+__device__ inline void taco_hip_spgemm_row_numeric_short(
+    int64_t n, int32_t col, int32_t jA, int32_t kB, const double* avals,
+    const double* bvals, int32_t c0, int32_t nc, double* cvals) {
+  double prod = 0;
+  if ((int)__lane_id() < n) prod = avals[jA] * bvals[kB];
+  taco_hip_spgemm_row_sum_store(n, col, prod, c0, nc, cvals);
 }
 
 __device__ inline void taco_hip_spgemm_row_numeric(

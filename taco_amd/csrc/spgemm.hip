@@ -30,9 +30,12 @@
 // Row kernels, one 64-lane wave per row, built on the row helpers of
 // include/taco_hip_rt.h (shared with modules emitted by CodeGen_HIP):
 //   short rows (<= 64 candidates): candidates in lanes, bitonic sort across
-//     the wave with shuffles.  The numeric phase sorts 64-bit keys
-//     (column << 32 | encounter index), so equal columns come out in
-//     encounter order and are summed sequentially in that order.
+//     the wave by register moves (DPP, lane-half swaps).  The numeric phase
+//     sorts (column, encounter index) keys, in 32 bits when the columns
+//     allow, so equal columns come out in encounter order and are summed
+//     sequentially in that order.  The hand-written short-row kernels take
+//     two rows a wave at a time, staging their loads together, and list
+//     longer rows for a second pass.
 //   long rows: a per-wave LDS hash set of up to TACO_HIP_ROW_CAP distinct
 //     columns, then an in-LDS bitonic sort; the numeric phase accumulates
 //     into LDS slots, adding candidates one at a time in encounter order
@@ -91,28 +94,129 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(
 }
 
 
+// Rows per wave taken together by the short-row kernels: their dependent
+// loads (A's row bounds, A's entries, B's row bounds, B's entries) are
+// issued stage by stage for all of them, so that many rows' loads are in
+// flight at once.
+constexpr int kRowsTogether = 2;
+
+// A wave's kRowsTogether short rows, rows base + q * stride (base and
+// stride wave-uniform), staged: each stage issues its loads for every row
+// before the next stage waits for them.  Every load reads a valid address
+// (indices clamped, A and B each holding at least one entry), so no load
+// sits under a branch, and lanes past a row's end mask what they loaded.
+// Lane t < la[q] holds A's entry t of row q; a row of more than 64
+// candidates (or 64 A entries) has total[q] = 65 and no candidates in
+// lanes; otherwise lane c < total[q] holds candidate c (encounter order):
+// its column col[q] and, with VALS, its product prod[q].
+template <int R, bool VALS>
+struct ShortRows {
+  bool real[R];
+  int32_t a0[R], la[R], total[R], col[R];
+  double prod[R];
+
+  __device__ void load(int64_t base, int64_t stride, int32_t m, const int32_t* apos,
+                       const int32_t* acrd, const double* avals, int32_t nnz_a,
+                       const int32_t* bpos, const int32_t* bcrd, const double* bvals,
+                       int32_t nnz_b) {
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int64_t i = base + q * stride;
+      real[q] = i < m;
+      const int64_t ic = real[q] ? i : m;
+      a0[q] = apos[ic];
+      la[q] = real[q] ? apos[ic + 1] - a0[q] : 0;
+    }
+    int32_t j[R];
+    double av[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      int32_t e = a0[q] + (lane < la[q] ? lane : la[q] - 1);
+      e = e < 0 ? 0 : (e < nnz_a ? e : nnz_a - 1);
+      j[q] = acrd[e];
+      av[q] = VALS ? avals[e] : 0.0;
+    }
+    int32_t b0[R], b1[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      if (lane >= la[q] || la[q] > 64) j[q] = 0;
+      b0[q] = bpos[j[q]];
+      b1[q] = bpos[j[q] + 1];
+    }
+    int32_t kB[R];
+    double a[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const bool mine = lane < la[q] && la[q] <= 64;
+      const int32_t len = (b1[q] - b0[q]) & (mine ? -1 : 0);
+      // B row lengths clamped to 65: the sums stay small and a total above
+      // 64 still reads as above 64.
+      const int32_t lc = len < 65 ? len : 65;
+      const int32_t inc = taco_hip_wave_scan_add_i32(lc);
+      total[q] = la[q] > 64 ? 65 : __builtin_amdgcn_readlane(inc, 63);
+      // Entry t holding candidate `lane`: the entries s < la - 1 whose
+      // inclusive sum is <= lane.
+      const int32_t tops = total[q] <= 64 ? la[q] - 1 : 0;
+      int t = 0;
+      for (int s2 = 0; s2 < tops; ++s2) t += __builtin_amdgcn_readlane(inc, s2) <= lane ? 1 : 0;
+      const int32_t excl = __shfl(inc - lc, t, 64);
+      const int32_t bt = __shfl(b0[q], t, 64);
+      a[q] = VALS ? __shfl(av[q], t, 64) : 0.0;
+      int32_t k = bt + (lane - excl);
+      kB[q] = k < 0 ? 0 : (k < nnz_b ? k : nnz_b - 1);
+    }
+    int32_t c[R];
+    double bv[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      c[q] = bcrd[kB[q]];
+      bv[q] = VALS ? bvals[kB[q]] : 0.0;
+    }
+    // Masks applied arithmetically, so that every lane uses what it loaded
+    // and the compiler keeps the loads where they are (a select would let
+    // it move them under a branch and wait there).  Columns are
+    // non-negative, so c | INT_MAX = INT_MAX; x * 1.0 is exact.
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const bool cand = total[q] <= 64 && lane < total[q];
+      col[q] = c[q] | (cand ? 0 : INT_MAX);
+      prod[q] = VALS ? a[q] * bv[q] * (cand ? 1.0 : 0.0) : 0.0;
+    }
+  }
+};
+
 // Numeric pass over the rows of at most 64 candidates, whose products stay
 // in lanes: this kernel uses no LDS, so twice as many waves stay resident as
 // with k_spgemm_numeric (whose long-row accumulators take 32 KB of LDS per
-// workgroup), and more rows' dependent loads are in flight.  Longer rows are
-// listed (long_rows, *nlong) for k_spgemm_numeric_listed.
+// workgroup), and each wave stages the loads of kRowsTogether rows at once.
+// Longer rows are listed (long_rows, *nlong) for k_spgemm_numeric_listed.
 __global__ __launch_bounds__(256) void k_spgemm_numeric_short(
     int32_t m, const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
-    const double* __restrict__ avals, const int32_t* __restrict__ bpos,
-    const int32_t* __restrict__ bcrd, const double* __restrict__ bvals,
+    const double* __restrict__ avals, int32_t nnz_a, const int32_t* __restrict__ bpos,
+    const int32_t* __restrict__ bcrd, const double* __restrict__ bvals, int32_t nnz_b,
     const int32_t* __restrict__ cpos, double* __restrict__ cvals,
     int32_t* __restrict__ long_rows, int32_t* __restrict__ nlong) {
+  const int lane = (int)__lane_id();
   const int64_t stride = (int64_t)gridDim.x * kWavesPerBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; i < m;
-       i += stride) {
-    const int32_t c0 = cpos[i], nc = cpos[i + 1] - c0;
-    int32_t col = INT_MAX, jA = 0, kB = 0;
-    const int64_t n = taco_hip_row_lanes(apos, acrd, bpos, bcrd, (int32_t)i, &col, &jA, &kB);
-    if (n < 0 || n > 64) {
-      if (__lane_id() == 0) long_rows[atomicAdd(nlong, 1)] = (int32_t)i;
-      continue;
+  const int64_t first = (int64_t)blockIdx.x * kWavesPerBlock +
+                        __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+  if (nnz_a == 0 || nnz_b == 0) return;  // C has no entries
+  for (int64_t base = first; base < m; base += kRowsTogether * stride) {
+    ShortRows<kRowsTogether, true> r;
+    r.load(base, stride, m, apos, acrd, avals, nnz_a, bpos, bcrd, bvals, nnz_b);
+#pragma unroll
+    for (int q = 0; q < kRowsTogether; ++q) {
+      if (!r.real[q]) continue;
+      const int64_t i = base + q * stride;
+      if (r.total[q] > 64) {
+        if (lane == 0) long_rows[atomicAdd(nlong, 1)] = (int32_t)i;
+        continue;
+      }
+      const int32_t c0 = cpos[i];
+      taco_hip_spgemm_row_sum_store(r.total[q], r.col[q], r.prod[q], c0, cpos[i + 1] - c0,
+                                    cvals);
     }
-    taco_hip_spgemm_row_numeric_short(n, col, jA, kB, avals, bvals, c0, nc, cvals);
   }
 }
 
@@ -191,29 +295,77 @@ __device__ inline int32_t row_sorted_set(const int32_t* apos, const int32_t* acr
   return cnt;
 }
 
-// Pass 2.  status[0] bit 4: the scratch holds fewer entries than the
-// candidate total, or the candidate offsets overflowed (nothing done);
-// status[1]: number of rows listed in long_rows (their count is left 0 for
-// the global-memory path).
-__global__ __launch_bounds__(256) void k_spgemm_symbolic_once(
+
+// Pass 2 over the rows of at most 64 candidates (no LDS, so more waves stay
+// resident): each row's sorted distinct columns go to its scratch region
+// and its distinct count to counts.  Longer rows are listed (mid_rows,
+// status[3]) for k_spgemm_symbolic_listed.  status[0] bit 4: the scratch
+// holds fewer entries than the candidate total, or the candidate offsets
+// overflowed (nothing done).
+__global__ __launch_bounds__(256) void k_spgemm_symbolic_short(
     int32_t m, const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
-    const int32_t* __restrict__ bpos, const int32_t* __restrict__ bcrd,
-    const int32_t* __restrict__ candoff, int64_t capacity, int32_t* __restrict__ scratch,
-    int32_t* __restrict__ counts, int32_t* __restrict__ long_rows,
-    int32_t* __restrict__ status) {
-  const int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  if (i >= m) return;
+    int32_t nnz_a, const int32_t* __restrict__ bpos, const int32_t* __restrict__ bcrd,
+    int32_t nnz_b, const int32_t* __restrict__ candoff, int64_t capacity,
+    int32_t* __restrict__ scratch, int32_t* __restrict__ counts,
+    int32_t* __restrict__ mid_rows, int32_t* __restrict__ status) {
+  const int lane = (int)__lane_id();
+  const int64_t first = (int64_t)blockIdx.x * kWavesPerBlock +
+                        __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
   // The candidate offsets overflowed int32 (status[2] bit 2: the caller
   // falls back to the two-pass form), or the scratch is too small: write
   // nothing.
   if ((status[2] & 2) || (int64_t)candoff[m] > capacity) {
-    if (i == 0 && __lane_id() == 0) atomicOr(status, 4);
+    if (first == 0 && lane == 0) atomicOr(status, 4);
     return;
   }
-  const int32_t c = row_sorted_set(apos, acrd, bpos, bcrd, (int32_t)i, scratch + candoff[i]);
-  if (__lane_id() == 0) {
-    if (c < 0) long_rows[atomicAdd(status + 1, 1)] = (int32_t)i;
-    counts[i] = c < 0 ? 0 : c;
+  const int64_t stride = (int64_t)gridDim.x * kWavesPerBlock;
+  if (nnz_a == 0 || nnz_b == 0) {
+    // No candidates anywhere (and nothing the staged loads may read).
+    for (int64_t i = first; i < m; i += stride)
+      if (lane == 0) counts[i] = 0;
+    return;
+  }
+  for (int64_t base = first; base < m; base += kRowsTogether * stride) {
+    ShortRows<kRowsTogether, false> r;
+    r.load(base, stride, m, apos, acrd, nullptr, nnz_a, bpos, bcrd, nullptr, nnz_b);
+#pragma unroll
+    for (int q = 0; q < kRowsTogether; ++q) {
+      if (!r.real[q]) continue;
+      const int64_t i = base + q * stride;
+      if (r.total[q] > 64) {
+        if (lane == 0) mid_rows[atomicAdd(status + 3, 1)] = (int32_t)i;
+        continue;
+      }
+      const int32_t v = (int32_t)taco_hip_wave_sort_u32((uint32_t)r.col[q]);
+      const int32_t prev = __shfl_up(v, 1, 64);
+      const bool head = v != INT_MAX && (lane == 0 || prev != v);
+      const uint64_t heads = __ballot(head);
+      if (head) scratch[candoff[i] + __popcll(heads & ((1ull << lane) - 1))] = v;
+      if (lane == 0) counts[i] = __popcll(heads);
+    }
+  }
+}
+
+// Pass 2 over the rows listed by k_spgemm_symbolic_short: the LDS row set
+// for up to TACO_HIP_ROW_CAP distinct columns; rows beyond it are listed in
+// long_rows (status[1]) for the global-memory sort, their count left 0.
+__global__ __launch_bounds__(256) void k_spgemm_symbolic_listed(
+    int32_t m, const int32_t* __restrict__ mid_rows, const int32_t* __restrict__ apos,
+    const int32_t* __restrict__ acrd, const int32_t* __restrict__ bpos,
+    const int32_t* __restrict__ bcrd, const int32_t* __restrict__ candoff,
+    int32_t* __restrict__ scratch, int32_t* __restrict__ counts,
+    int32_t* __restrict__ long_rows, int32_t* __restrict__ status) {
+  if (status[0] & 4) return;
+  const int32_t count = status[3];
+  const int64_t stride = (int64_t)gridDim.x * kWavesPerBlock;
+  for (int64_t r = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; r < count;
+       r += stride) {
+    const int32_t i = mid_rows[r];
+    const int32_t c = row_sorted_set(apos, acrd, bpos, bcrd, i, scratch + candoff[i]);
+    if (__lane_id() == 0) {
+      if (c < 0) long_rows[atomicAdd(status + 1, 1)] = i;
+      counts[i] = c < 0 ? 0 : c;
+    }
   }
 }
 
@@ -518,6 +670,15 @@ int device_cus() {
   return cus;
 }
 
+// Resident grid of the short-row kernels (4 waves per workgroup, each
+// taking kRowsTogether rows per step), no larger than the rows need.
+unsigned short_rows_grid(int32_t m) {
+  const int64_t need = ((int64_t)m + kWavesPerBlock * kRowsTogether - 1) /
+                       (kWavesPerBlock * kRowsTogether);
+  const int64_t resident = (int64_t)kShortBlocksPerCu * device_cus();
+  return (unsigned)(need < resident ? (need > 0 ? need : 1) : resident);
+}
+
 // Rows listed by the symbolic pass (more than kHashCap distinct columns):
 // candidates written out, sorted per row in global memory, deduplicated into
 // their scratch regions with their distinct counts.
@@ -576,15 +737,17 @@ int assemble_one_sort(const CsrView& a, const CsrView& b, taco_tensor_t* C, hipS
   const int32_t m = a.m;
   // status[0]: flags (2: nnz(C) beyond int32, 4: scratch too small);
   // status[1]: long rows listed; status[2]: candidate-total flags (2).
-  char* ws = (char*)workspace(256 + sizeof(int32_t) * 2 * ((size_t)m + 1), 2);
+  char* ws = (char*)workspace(256 + sizeof(int32_t) * 3 * ((size_t)m + 1), 2);
   int32_t* candoff = (int32_t*)workspace(sizeof(int32_t) * ((size_t)m + 1), kSlotCandOff);
   if (!ws || !candoff) return 1;
   int32_t* status = (int32_t*)ws;
   int32_t* counts = (int32_t*)(ws + 256);
   int32_t* long_rows = counts + m + 1;
+  int32_t* mid_rows = long_rows + m + 1;
   int32_t* cpos = (int32_t*)taco_hip_unified_alloc(sizeof(int32_t) * ((size_t)m + 1));
   if (!cpos) return 1;
-  const unsigned grid = (unsigned)(((int64_t)m + kWavesPerBlock - 1) / kWavesPerBlock);
+  const unsigned short_grid = short_rows_grid(m);
+  const unsigned listed_grid = (unsigned)taco_hip_row_grid(m);
   int rc = 0;
   do {
     if (hipMemsetAsync(status, 0, 4 * sizeof(int32_t), st) != hipSuccess) {
@@ -604,14 +767,19 @@ int assemble_one_sort(const CsrView& a, const CsrView& b, taco_tensor_t* C, hipS
     cap_bytes = workspace_size(kSlotScratch);
     int32_t host[5] = {0, 0, 0, 0, 0};  // status[0..2], candoff[m], cpos[m]
     for (int attempt = 0; attempt < 2; ++attempt) {
-      if (hipMemsetAsync(status, 0, 2 * sizeof(int32_t), st) != hipSuccess) {
+      if (hipMemsetAsync(status, 0, 2 * sizeof(int32_t), st) != hipSuccess ||
+          hipMemsetAsync(status + 3, 0, sizeof(int32_t), st) != hipSuccess) {
         rc = fail("spgemm: memset failed");
         break;
       }
-      if (m > 0)
-        k_spgemm_symbolic_once<<<grid, 256, 0, st>>>(
-            m, a.pos, a.crd, b.pos, b.crd, candoff, (int64_t)(cap_bytes / sizeof(int32_t)),
-            scratch, counts, long_rows, status);
+      if (m > 0) {
+        k_spgemm_symbolic_short<<<short_grid, 256, 0, st>>>(
+            m, a.pos, a.crd, (int32_t)a.nnz, b.pos, b.crd, (int32_t)b.nnz, candoff,
+            (int64_t)(cap_bytes / sizeof(int32_t)), scratch, counts, mid_rows, status);
+        k_spgemm_symbolic_listed<<<listed_grid, 256, 0, st>>>(
+            m, mid_rows, a.pos, a.crd, b.pos, b.crd, candoff, scratch, counts, long_rows,
+            status);
+      }
       if ((rc = device_counts_to_pos(m, counts, cpos, status, st))) break;
       if (hipMemcpyAsync(host, status, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
               hipSuccess ||
@@ -708,12 +876,9 @@ extern "C" int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,
   int32_t* nlong = (int32_t*)ws;
   int32_t* long_rows = (int32_t*)(ws + 256);
   k_spgemm_zero_count<<<1, 64, 0, st>>>(nlong);
-  const int64_t short_grid = (a.m + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int64_t resident = (int64_t)kShortBlocksPerCu * device_cus();
-  k_spgemm_numeric_short<<<(unsigned)(short_grid < resident ? short_grid : resident), 256, 0,
-                           st>>>(a.m, a.pos, a.crd, (const double*)a.vals, b.pos, b.crd,
-                                 (const double*)b.vals, c.pos, (double*)c.vals, long_rows,
-                                 nlong);
+  k_spgemm_numeric_short<<<short_rows_grid(a.m), 256, 0, st>>>(
+      a.m, a.pos, a.crd, (const double*)a.vals, (int32_t)a.nnz, b.pos, b.crd,
+      (const double*)b.vals, (int32_t)b.nnz, c.pos, (double*)c.vals, long_rows, nlong);
   k_spgemm_numeric_listed<<<(unsigned)taco_hip_row_grid(a.m), 256, 0, st>>>(
       long_rows, nlong, a.pos, a.crd, (const double*)a.vals, b.pos, b.crd,
       (const double*)b.vals, c.pos, c.crd, (double*)c.vals);
