@@ -34,7 +34,7 @@
 //     sorts (column, encounter index) keys, in 32 bits when the columns
 //     allow, so equal columns come out in encounter order and are summed
 //     sequentially in that order.  The hand-written short-row kernels take
-//     two rows a wave at a time, staging their loads together, and list
+//     three rows a wave at a time, staging their loads together, and list
 //     longer rows for a second pass.
 //   long rows: a per-wave LDS hash set of up to TACO_HIP_ROW_CAP distinct
 //     columns, then an in-LDS bitonic sort; the numeric phase accumulates
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(
 // loads (A's row bounds, A's entries, B's row bounds, B's entries) are
 // issued stage by stage for all of them, so that many rows' loads are in
 // flight at once.
-constexpr int kRowsTogether = 2;
+constexpr int kRowsTogether = 3;
 
 // A wave's kRowsTogether short rows, rows base + q * stride (base and
 // stride wave-uniform), staged: each stage issues its loads for every row
