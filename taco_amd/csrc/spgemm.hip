@@ -433,6 +433,7 @@ __global__ __launch_bounds__(256) void k_spgemm_long_dedup(
 // stride the range, each finding its row by binary search in LDS, so the
 // stores are coalesced whatever the row lengths.
 constexpr int kCompactRows = 256;
+constexpr int kCompactUnroll = 4;
 __global__ __launch_bounds__(256) void k_spgemm_compact(
     int32_t m, const int32_t* __restrict__ candoff, const int32_t* __restrict__ cpos,
     const int32_t* __restrict__ scratch, int32_t* __restrict__ ccrd) {
@@ -444,14 +445,36 @@ __global__ __launch_bounds__(256) void k_spgemm_compact(
   for (int q = threadIdx.x; q < nr; q += blockDim.x) s_src[q] = candoff[r0 + q];
   __syncthreads();
   const int32_t e0 = s_pos[0], e1 = s_pos[nr];
-  for (int32_t e = e0 + (int32_t)threadIdx.x; e < e1; e += (int32_t)blockDim.x) {
-    // the row r of output entry e: largest r with s_pos[r] <= e
-    int lo = 0, hi = nr - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_pos[mid] <= e) lo = mid; else hi = mid - 1;
+  // kCompactUnroll entries per thread at a time, their row searches
+  // interleaved and their loads in flight together; entries past the end
+  // are clamped to the last entry of the range.
+  for (int32_t e = e0 + (int32_t)threadIdx.x; e < e1;
+       e += kCompactUnroll * (int32_t)blockDim.x) {
+    int32_t ee[kCompactUnroll], lo[kCompactUnroll];
+#pragma unroll
+    for (int u = 0; u < kCompactUnroll; ++u) {
+      const int32_t x = e + u * (int32_t)blockDim.x;
+      ee[u] = x < e1 ? x : e1 - 1;
+      lo[u] = 0;
     }
-    __builtin_nontemporal_store(scratch[s_src[lo] + (e - s_pos[lo])], ccrd + e);
+    // The row of entry ee: the largest r < nr with s_pos[r] <= ee.
+#pragma unroll
+    for (int step = kCompactRows / 2; step > 0; step >>= 1) {
+#pragma unroll
+      for (int u = 0; u < kCompactUnroll; ++u) {
+        const int c = lo[u] + step;
+        const int32_t pv = s_pos[c < nr ? c : nr - 1];  // read unconditionally
+        lo[u] = ((c < nr) & (pv <= ee[u])) ? c : lo[u];
+      }
+    }
+    int32_t v[kCompactUnroll];
+#pragma unroll
+    for (int u = 0; u < kCompactUnroll; ++u)
+      v[u] = scratch[s_src[lo[u]] + (ee[u] - s_pos[lo[u]])];
+    // Entries past the end store the last entry's own value again (ee is
+    // clamped), so every lane stores and no load waits under a branch.
+#pragma unroll
+    for (int u = 0; u < kCompactUnroll; ++u) __builtin_nontemporal_store(v[u], ccrd + ee[u]);
   }
 }
 
