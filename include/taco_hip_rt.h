@@ -471,6 +471,56 @@ __device__ inline uint32_t taco_hip_bitonic_merge_u32(uint32_t v) {
   return taco_hip_bitonic_step_u32<K, 1>(v);
 }
 
+/* The 21 compare-exchange steps' lane masks of the wave bitonic sort (all
+ * ones where the lane keeps the smaller value), built once per kernel and
+ * held in vector registers: as scalar lane masks they would be 42 SGPRs,
+ * beyond what a kernel keeps without spilling them. */
+struct TacoHipSortMasks {
+  uint32_t m[21];
+  __device__ TacoHipSortMasks() {
+    const int lane = (int)__lane_id();
+    int s = 0;
+#pragma unroll
+    for (int K = 2; K <= 64; K <<= 1) {
+#pragma unroll
+      for (int J = K >> 1; J > 0; J >>= 1) {
+        uint32_t keep = (((lane & K) == 0) == ((lane & J) == 0)) ? 0xFFFFFFFFu : 0u;
+        asm volatile("" : "+v"(keep)); /* keep it a per-lane value */
+        m[s++] = keep;
+      }
+    }
+  }
+};
+
+template <int K, int J>
+__device__ inline uint32_t taco_hip_bitonic_step_masked(uint32_t v, uint32_t keep_min) {
+  const uint32_t p = taco_hip_lane_xor_u32<J>(v);
+  const uint32_t lo = p < v ? p : v, hi = p > v ? p : v;
+  return hi ^ ((lo ^ hi) & keep_min); /* one three-input bit operation */
+}
+
+template <int K>
+__device__ inline uint32_t taco_hip_bitonic_merge_masked(uint32_t v, const uint32_t* m) {
+  /* m: this merge's steps, J = K/2 down to 1 */
+  int s = 0;
+  if constexpr (K >= 64) v = taco_hip_bitonic_step_masked<K, 32>(v, m[s++]);
+  if constexpr (K >= 32) v = taco_hip_bitonic_step_masked<K, 16>(v, m[s++]);
+  if constexpr (K >= 16) v = taco_hip_bitonic_step_masked<K, 8>(v, m[s++]);
+  if constexpr (K >= 8) v = taco_hip_bitonic_step_masked<K, 4>(v, m[s++]);
+  if constexpr (K >= 4) v = taco_hip_bitonic_step_masked<K, 2>(v, m[s++]);
+  return taco_hip_bitonic_step_masked<K, 1>(v, m[s]);
+}
+
+/* taco_hip_wave_sort_u32 with the step masks given (same result). */
+__device__ inline uint32_t taco_hip_wave_sort_u32(uint32_t v, const TacoHipSortMasks& sm) {
+  v = taco_hip_bitonic_merge_masked<2>(v, sm.m + 0);
+  v = taco_hip_bitonic_merge_masked<4>(v, sm.m + 1);
+  v = taco_hip_bitonic_merge_masked<8>(v, sm.m + 3);
+  v = taco_hip_bitonic_merge_masked<16>(v, sm.m + 6);
+  v = taco_hip_bitonic_merge_masked<32>(v, sm.m + 10);
+  return taco_hip_bitonic_merge_masked<64>(v, sm.m + 15);
+}
+
 /* taco_hip_wave_sort for 32-bit unsigned keys, every exchange a register
  * move (taco_hip_lane_xor_u32). */
 __device__ inline uint32_t taco_hip_wave_sort_u32(uint32_t v) {
@@ -815,7 +865,8 @@ __device__ inline void taco_hip_spgemm_row_numeric_global(
  * summed per column in encounter order; the row's values go to
  * cvals[c0 .. c0 + nc).  Every lane of the wave must call it. */
 __device__ inline void taco_hip_spgemm_row_sum_store(int64_t n, int32_t col, double prod,
-                                                     int32_t c0, int32_t nc, double* cvals) {
+                                                     int32_t c0, int32_t nc, double* cvals,
+                                                     const TacoHipSortMasks* sm = nullptr) {
   const int lane = (int)__lane_id();
   (void)n;
   /* Key = column, then encounter index (which also names the lane): in
@@ -823,7 +874,7 @@ __device__ inline void taco_hip_spgemm_row_sum_store(int64_t n, int32_t col, dou
   int32_t scol, src;
   if (__ballot(col != INT_MAX && col >= (1 << 26)) == 0) {
     uint32_t key = col == INT_MAX ? 0xFFFFFFFFu : ((uint32_t)col << 6) | (uint32_t)lane;
-    key = taco_hip_wave_sort_u32(key);
+    key = sm ? taco_hip_wave_sort_u32(key, *sm) : taco_hip_wave_sort_u32(key);
     scol = key == 0xFFFFFFFFu ? INT_MAX : (int32_t)(key >> 6);
     src = (int32_t)(key & 63);
   } else {

@@ -73,27 +73,6 @@ __global__ __launch_bounds__(256) void k_spgemm_symbolic(
   if (!FILL && __lane_id() == 0) counts[i] = c;
 }
 
-// Numeric pass on the assembled pattern.  Rows are strided over a grid the
-// device holds at once (taco_hip_row_grid, 4 workgroups per CU; the kernel
-// fits 5): one workgroup per four rows left about 4
-// waves resident per CU, since a row takes only a few thousand cycles
-// (0.56 -> 0.51 ms at 500k rows).  The symbolic passes measured faster with
-// one workgroup per four rows, where running waves stream through
-// neighbouring rows (0.63 against 0.68-0.73 ms).
-__global__ __launch_bounds__(256) void k_spgemm_numeric(
-    int32_t m, const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
-    const double* __restrict__ avals, const int32_t* __restrict__ bpos,
-    const int32_t* __restrict__ bcrd, const double* __restrict__ bvals,
-    const int32_t* __restrict__ cpos, const int32_t* __restrict__ ccrd,
-    double* __restrict__ cvals) {
-  const int64_t stride = (int64_t)gridDim.x * kWavesPerBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; i < m;
-       i += stride)
-    taco_hip_spgemm_row_numeric(apos, acrd, avals, bpos, bcrd, bvals, (int32_t)i,
-                                cpos, ccrd, cvals);
-}
-
-
 // Rows per wave taken together by the short-row kernels: their dependent
 // loads (A's row bounds, A's entries, B's row bounds, B's entries) are
 // issued stage by stage for all of them, so that many rows' loads are in
@@ -188,9 +167,10 @@ struct ShortRows {
 
 // Numeric pass over the rows of at most 64 candidates, whose products stay
 // in lanes: this kernel uses no LDS, so twice as many waves stay resident as
-// with k_spgemm_numeric (whose long-row accumulators take 32 KB of LDS per
-// workgroup), and each wave stages the loads of kRowsTogether rows at once.
-// Longer rows are listed (long_rows, *nlong) for k_spgemm_numeric_listed.
+// with the general row kernel (whose long-row accumulators take 32 KB of
+// LDS per workgroup), and each wave stages the loads of kRowsTogether rows
+// at once.  Longer rows are listed (long_rows, *nlong) for
+// k_spgemm_numeric_listed, the general row kernel over the list.
 __global__ __launch_bounds__(256) void k_spgemm_numeric_short(
     int32_t m, const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
     const double* __restrict__ avals, int32_t nnz_a, const int32_t* __restrict__ bpos,
@@ -202,6 +182,7 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric_short(
   const int64_t first = (int64_t)blockIdx.x * kWavesPerBlock +
                         __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
   if (nnz_a == 0 || nnz_b == 0) return;  // C has no entries
+  const TacoHipSortMasks sm;
   for (int64_t base = first; base < m; base += kRowsTogether * stride) {
     ShortRows<kRowsTogether, true> r;
     r.load(base, stride, m, apos, acrd, avals, nnz_a, bpos, bcrd, bvals, nnz_b);
@@ -215,7 +196,7 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric_short(
       }
       const int32_t c0 = cpos[i];
       taco_hip_spgemm_row_sum_store(r.total[q], r.col[q], r.prod[q], c0, cpos[i + 1] - c0,
-                                    cvals);
+                                    cvals, &sm);
     }
   }
 }
@@ -325,6 +306,7 @@ __global__ __launch_bounds__(256) void k_spgemm_symbolic_short(
       if (lane == 0) counts[i] = 0;
     return;
   }
+  const TacoHipSortMasks sm;
   for (int64_t base = first; base < m; base += kRowsTogether * stride) {
     ShortRows<kRowsTogether, false> r;
     r.load(base, stride, m, apos, acrd, nullptr, nnz_a, bpos, bcrd, nullptr, nnz_b);
@@ -336,7 +318,7 @@ __global__ __launch_bounds__(256) void k_spgemm_symbolic_short(
         if (lane == 0) mid_rows[atomicAdd(status + 3, 1)] = (int32_t)i;
         continue;
       }
-      const int32_t v = (int32_t)taco_hip_wave_sort_u32((uint32_t)r.col[q]);
+      const int32_t v = (int32_t)taco_hip_wave_sort_u32((uint32_t)r.col[q], sm);
       const int32_t prev = __shfl_up(v, 1, 64);
       const bool head = v != INT_MAX && (lane == 0 || prev != v);
       const uint64_t heads = __ballot(head);
