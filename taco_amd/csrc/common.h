@@ -34,6 +34,10 @@ int finish(const char* what);
 void* workspace(size_t bytes, int slot);
 // Bytes currently held by a workspace slot on the current device (0: none).
 size_t workspace_size(int slot);
+// Reads n <= 64 int32 values from device memory `src` into `dst` with one
+// copy through a page-locked host buffer (one per host thread), after the
+// stream's queued work.  Returns 0 or an error code (recorded).
+int read_back_i32(const int32_t* src, int n, int32_t* dst, hipStream_t st);
 
 // Value types carried by descriptors (csize, in bits).
 enum class VType { F64, F32, BF16, Invalid };

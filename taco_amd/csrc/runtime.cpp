@@ -133,6 +133,22 @@ size_t workspace_size(int slot) {
   return w.ptr ? w.bytes : 0;
 }
 
+int read_back_i32(const int32_t* src, int n, int32_t* dst, hipStream_t st) {
+  TH_REQUIRE(n >= 0 && n <= 64, "read back: %d values", n);
+  // Page-locked, so the copy is one DMA of the bytes rather than a staged
+  // copy through a driver buffer; kept for the thread's lifetime.
+  thread_local int32_t* pinned = nullptr;
+  if (!pinned) {
+    void* p = nullptr;
+    TH_CHECK(hipHostMalloc(&p, 64 * sizeof(int32_t), hipHostMallocDefault));
+    pinned = (int32_t*)p;
+  }
+  TH_CHECK(hipMemcpyAsync(pinned, src, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
+  TH_CHECK(hipStreamSynchronize(st));
+  std::memcpy(dst, pinned, sizeof(int32_t) * (size_t)n);
+  return 0;
+}
+
 int view_csr(const taco_tensor_t* t, const char* name, CsrView* out) {
   TH_REQUIRE(t, "%s: null descriptor", name);
   TH_REQUIRE(t->order == 2, "%s: expected a matrix, order is %d", name,

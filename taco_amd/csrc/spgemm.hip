@@ -221,6 +221,21 @@ __global__ void k_spgemm_zero_count(int32_t* __restrict__ count) {
   if (threadIdx.x == 0) *count = 0;
 }
 
+// out[0..4] = status[0..2], *candoff_m, *cpos_m: what the host reads back
+// after the row-set pass, gathered for one copy.
+__global__ void k_spgemm_gather_result(const int32_t* __restrict__ status,
+                                       const int32_t* __restrict__ candoff_m,
+                                       const int32_t* __restrict__ cpos_m,
+                                       int32_t* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    out[0] = status[0];
+    out[1] = status[1];
+    out[2] = status[2];
+    out[3] = *candoff_m;
+    out[4] = *cpos_m;
+  }
+}
+
 // ---- one-sort assemble ----
 
 // cand[i] = candidates of row i (saturated at INT_MAX; the prefix sum then
@@ -786,16 +801,8 @@ int assemble_one_sort(const CsrView& a, const CsrView& b, taco_tensor_t* C, hipS
             status);
       }
       if ((rc = device_counts_to_pos(m, counts, cpos, status, st))) break;
-      if (hipMemcpyAsync(host, status, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
-              hipSuccess ||
-          hipMemcpyAsync(&host[3], candoff + m, sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
-              hipSuccess ||
-          hipMemcpyAsync(&host[4], cpos + m, sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
-              hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess) {
-        rc = fail("spgemm: reading nnz(C) failed");
-        break;
-      }
+      k_spgemm_gather_result<<<1, 64, 0, st>>>(status, candoff + m, cpos + m, status + 8);
+      if ((rc = read_back_i32(status + 8, 5, host, st))) break;
       if ((host[2] & 2) || !(host[0] & 4)) break;
       // The scratch was too small for the candidate total: grow, run again.
       cap_bytes = sizeof(int32_t) * (size_t)host[3];
@@ -819,14 +826,8 @@ int assemble_one_sort(const CsrView& a, const CsrView& b, taco_tensor_t* C, hipS
         break;
       }
       if ((rc = device_counts_to_pos(m, counts, cpos, status, st))) break;
-      if (hipMemcpyAsync(host, status, sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
-              hipSuccess ||
-          hipMemcpyAsync(&host[4], cpos + m, sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
-              hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess) {
-        rc = fail("spgemm: reading nnz(C) failed");
-        break;
-      }
+      k_spgemm_gather_result<<<1, 64, 0, st>>>(status, candoff + m, cpos + m, status + 8);
+      if ((rc = read_back_i32(status + 8, 5, host, st))) break;
     }
     if (host[0] & 2) {
       rc = fail("spgemm: nnz(C) exceeds the int32 index range");
