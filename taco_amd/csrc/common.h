@@ -38,6 +38,9 @@ size_t workspace_size(int slot);
 // copy through a page-locked host buffer (one per host thread), after the
 // stream's queued work.  Returns 0 or an error code (recorded).
 int read_back_i32(const int32_t* src, int n, int32_t* dst, hipStream_t st);
+// Zeroes `bytes` (a multiple of 4) at the 4-byte-aligned p with a kernel on
+// st: unlike hipMemsetAsync, safe inside a HIP graph capture.
+int zero_fill(void* p, size_t bytes, hipStream_t st);
 
 // Value types carried by descriptors (csize, in bits).
 enum class VType { F64, F32, BF16, Invalid };

@@ -243,9 +243,13 @@ __global__ void k_spgemm_gather_result(const int32_t* __restrict__ status,
 // A entries, so a wave covers 64 / kCandLanes rows and a row's loads are in
 // flight together.
 constexpr int kCandLanes = 8;
+// It also clears the assemble's status words (status[0..15]), which no
+// kernel reads before the prefix sum that follows it.
 __global__ __launch_bounds__(256) void k_spgemm_row_cands(
     int32_t m, const int32_t* __restrict__ apos, const int32_t* __restrict__ acrd,
-    const int32_t* __restrict__ bpos, int32_t* __restrict__ cand) {
+    const int32_t* __restrict__ bpos, int32_t* __restrict__ cand,
+    int32_t* __restrict__ status) {
+  if (blockIdx.x == 0 && threadIdx.x < 16) status[threadIdx.x] = 0;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t i = t / kCandLanes;
   const int sub = (int)(t % kCandLanes);
@@ -770,13 +774,15 @@ int assemble_one_sort(const CsrView& a, const CsrView& b, taco_tensor_t* C, hipS
   const unsigned listed_grid = (unsigned)taco_hip_row_grid(m);
   int rc = 0;
   do {
-    if (hipMemsetAsync(status, 0, 4 * sizeof(int32_t), st) != hipSuccess) {
+    // The status words are cleared by k_spgemm_row_cands (a memset costs a
+    // runtime call of its own).
+    if (m > 0)
+      k_spgemm_row_cands<<<(unsigned)(((int64_t)m * kCandLanes + 255) / 256), 256, 0, st>>>(
+          m, a.pos, a.crd, b.pos, counts, status);
+    else if (hipMemsetAsync(status, 0, 16 * sizeof(int32_t), st) != hipSuccess) {
       rc = fail("spgemm: memset failed");
       break;
     }
-    if (m > 0)
-      k_spgemm_row_cands<<<(unsigned)(((int64_t)m * kCandLanes + 255) / 256), 256, 0, st>>>(
-          m, a.pos, a.crd, b.pos, counts);
     if ((rc = device_counts_to_pos(m, counts, candoff, status + 2, st))) break;
     size_t cap_bytes = workspace_size(kSlotScratch);
     int32_t* scratch = (int32_t*)workspace(cap_bytes ? cap_bytes : 4096, kSlotScratch);
@@ -787,8 +793,9 @@ int assemble_one_sort(const CsrView& a, const CsrView& b, taco_tensor_t* C, hipS
     cap_bytes = workspace_size(kSlotScratch);
     int32_t host[5] = {0, 0, 0, 0, 0};  // status[0..2], candoff[m], cpos[m]
     for (int attempt = 0; attempt < 2; ++attempt) {
-      if (hipMemsetAsync(status, 0, 2 * sizeof(int32_t), st) != hipSuccess ||
-          hipMemsetAsync(status + 3, 0, sizeof(int32_t), st) != hipSuccess) {
+      if (attempt > 0 &&
+          (hipMemsetAsync(status, 0, 2 * sizeof(int32_t), st) != hipSuccess ||
+           hipMemsetAsync(status + 3, 0, sizeof(int32_t), st) != hipSuccess)) {
         rc = fail("spgemm: memset failed");
         break;
       }

@@ -373,7 +373,7 @@ int run_spmm(const CsrView& a, int32_t K, const TA* B, TC* C,
   }
   if (algo == TACO_HIP_NNZ_SPLIT) {
     int npt = s.items_per_thread > 0 ? s.items_per_thread : 8;
-    TH_CHECK(hipMemsetAsync(C, 0, sizeof(TC) * (size_t)a.m * K, st));
+    if (int rc = zero_fill(C, sizeof(TC) * (size_t)a.m * K, st)) return rc;
     if (a.nnz == 0) return 0;
     int64_t teams = (a.nnz + npt - 1) / npt;
     unsigned grid = (unsigned)((teams + teams_per_block - 1) / teams_per_block);

@@ -35,8 +35,11 @@ constexpr int kTileMax = 1024;  // entries per MFMA task
 
 __global__ __launch_bounds__(256) void k_mfma_parts(
     int32_t m, int32_t K, const int32_t* __restrict__ pos,
-    int32_t* __restrict__ parts, float* __restrict__ C) {
+    int32_t* __restrict__ parts, float* __restrict__ C, int32_t* __restrict__ status) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // The prefix sum that follows flags overflow in *status: cleared here, in
+  // a kernel rather than a memset, so the call can be captured in a graph.
+  if (g == 0) *status = 0;
   const int64_t ngroups = ((int64_t)m + 15) / 16;
   if (g >= ngroups) return;
   const int64_t r0 = g * 16;
@@ -173,8 +176,7 @@ int spmm_mfma_bf16(const CsrView& a, int32_t K, const void* B, float* C,
   int32_t* status = (int32_t*)ws;
   int32_t* parts = (int32_t*)(ws + 256);
   int32_t* part_pos = parts + ngroups + 1;
-  TH_CHECK(hipMemsetAsync(status, 0, sizeof(int32_t), st));
-  k_mfma_parts<<<(ngroups + 255) / 256, 256, 0, st>>>(a.m, K, a.pos, parts, C);
+  k_mfma_parts<<<(ngroups + 255) / 256, 256, 0, st>>>(a.m, K, a.pos, parts, C, status);
   if (int rc = device_counts_to_pos(ngroups, parts, part_pos, status, st))
     return rc;
   const int64_t max_tasks = (int64_t)ngroups + (a.nnz + kTileMax - 1) / kTileMax;

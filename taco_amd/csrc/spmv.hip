@@ -325,7 +325,7 @@ int run_spmv(const CsrView& a, const T* x, T* y, const taco_hip_schedule& s,
     int bs = s.block_size > 0 ? s.block_size : 256;
     int k = s.items_per_thread > 0 ? s.items_per_thread : 8;
     TH_REQUIRE(bs % kWave == 0 && bs <= 1024, "spmv: bad block size %d", bs);
-    TH_CHECK(hipMemsetAsync(y, 0, sizeof(T) * (size_t)a.m, st));
+    if (int rc = zero_fill(y, sizeof(T) * (size_t)a.m, st)) return rc;
     if (a.nnz == 0) return 0;
     int64_t threads = (a.nnz + k - 1) / k;
     unsigned grid = (unsigned)((threads + bs - 1) / bs);
@@ -384,7 +384,24 @@ int run_spmv(const CsrView& a, const T* x, T* y, const taco_hip_schedule& s,
   return fail("spmv: schedule algo %d not available for SpMV", algo);
 }
 
+__global__ __launch_bounds__(256) void k_zero_fill(uint32_t* __restrict__ p, int64_t words) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride)
+    p[i] = 0u;
+}
+
 }  // namespace
+
+int zero_fill(void* p, size_t bytes, hipStream_t st) {
+  TH_REQUIRE(bytes % 4 == 0 && ((uintptr_t)p % 4) == 0, "zero fill: %zu bytes at %p", bytes, p);
+  const int64_t words = (int64_t)(bytes / 4);
+  if (words == 0) return 0;
+  const int64_t blocks = (words + 255) / 256;
+  k_zero_fill<<<(unsigned)(blocks < 8192 ? blocks : 8192), 256, 0, st>>>((uint32_t*)p, words);
+  TH_LAUNCH_CHECK("k_zero_fill");
+  return 0;
+}
+
 }  // namespace taco_hip
 
 using namespace taco_hip;
