@@ -277,6 +277,7 @@ def bench_spmv(args, A, x, name, workload_desc, spmm=None):
     torch.cuda.synchronize()
     plan_ms = (time.perf_counter() - t0) * 1e3
     plan_info = plan.info()
+    plan_shape = plan.shape()
     log(f"gpu: plan {plan_ms:.1f} ms {plan_info}; {args.warmup} warmup + {args.steps} steps ...")
     # Events around the main kernel of every timed step (same stream).
     ms, wall_ms = timer.run(lambda: plan.compute(ydsc, xdsc, args.unroll), args.steps, args.warmup,
@@ -365,9 +366,10 @@ def bench_spmv(args, A, x, name, workload_desc, spmm=None):
         "dtype": "f64", "data": "synthetic (seed-generated)",
         "config": {"workload": name, "desc": workload_desc, "m": A.m, "n": A.n,
                    "nnz": A.nnz, "schedule": f"slab-tiled plan parts={plan_info['parts']} "
-                                             f"slab_width={plan_info['slab_width']} unroll={args.unroll or 8} "
-                                             f"threads=256, interleaved entries, segments "
-                                             f"ordered by column",
+                                             f"slab_width={plan_info['slab_width']} "
+                                             f"unroll={args.unroll or plan_shape['unroll']} "
+                                             f"threads={plan_shape['threads']}, interleaved "
+                                             f"entries, segments ordered by column",
                    "parallelism": "single GPU"},
         "pct_of_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS,
         "gflops": 2.0 * A.nnz / (ms * 1e-3) / 1e9,

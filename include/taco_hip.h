@@ -182,7 +182,8 @@ int taco_hip_spgemm_compute(taco_tensor_t* C, taco_tensor_t* A,
 
 /* Slab-tiled SpMV plan (f64).  A storage conversion of a CSR matrix for
  * SpMV whose x exceeds one XCD's L2: columns split into slabs of slab_width
- * (<= 2^18; 0 = 2^17, 1 MB of x), rows into `parts` workgroup parts (0 = 512)
+ * (<= 2^17; 0 = 2^17, 1 MB of x), rows into `parts` workgroup parts (0 = one
+ * per CU, or a multiple of the CU count when rows would crowd the LDS)
  * whose entries are reordered slab by slab, and by column inside each
  * (part, slab) segment, so that resident workgroups sweep the slabs in step
  * and gather x from L2, a wave's lanes from nearby columns.  compute's
@@ -211,6 +212,11 @@ int taco_hip_spmv_plan_info(const taco_hip_spmv_plan* plan, int64_t* info);
  * atomics (4 by default). */
 int taco_hip_spmv_plan_configure(taco_hip_spmv_plan* plan, int32_t threads,
                                  int32_t form);
+/* The main kernel's launch shape: workgroup size (by default 512 when each
+ * CU holds one part, else 256), kernel form, and the entries per thread
+ * used when compute's `unroll` is 0. */
+int taco_hip_spmv_plan_shape(const taco_hip_spmv_plan* plan, int32_t* threads,
+                             int32_t* form, int32_t* unroll);
 /* Kernel timing: HIP events bracket the main kernel (k_slab_spmv) of the
  * next `launches` compute calls on their stream (0 stops and releases).
  * plan_kernel_time waits for the last recorded call and returns the summed

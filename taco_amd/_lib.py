@@ -94,6 +94,8 @@ _SIGS = {
     "taco_hip_spmv_plan_info": (c_int, [c_void_p, POINTER(c_int64)]),
     "taco_hip_spmv_plan_time_kernel": (c_int, [c_void_p, c_int32]),
     "taco_hip_spmv_plan_configure": (c_int, [c_void_p, c_int32, c_int32]),
+    "taco_hip_spmv_plan_shape": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32),
+                                         POINTER(c_int32)]),
     "taco_hip_spmv_plan_kernel_time": (c_int, [c_void_p, POINTER(c_double), POINTER(c_int64)]),
     "taco_hip_batch_create": (c_int, [c_int32, c_int32, c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_int32, c_int32, c_void_p, c_void_p,
@@ -256,6 +258,13 @@ class SpmvPlan:
         """Workgroup size (256 or 512) and kernel form: 2 interleaved entries
         (the default), 0 consecutive entries per thread, 1 wave-summed runs."""
         check(lib().taco_hip_spmv_plan_configure(self._p, threads, form), "plan configure")
+
+    def shape(self):
+        """The main kernel's launch shape: threads, form, default unroll."""
+        t, f, u = c_int32(), c_int32(), c_int32()
+        check(lib().taco_hip_spmv_plan_shape(self._p, ctypes.byref(t), ctypes.byref(f),
+                                             ctypes.byref(u)), "plan shape")
+        return {"threads": t.value, "form": f.value, "unroll": u.value}
 
     def time_kernel(self, launches):
         """Bracket the main kernel of the next `launches` computes with events."""

@@ -19,7 +19,7 @@
 // r_{p+1}; carries are added in part order by a fix-up kernel, as in the
 // merge-path kernel.
 //
-// Packed entry: (local row << 18) | (column - slab * slab_width), so an
+// Packed entry: (local row << 17) | (column - slab * slab_width), so an
 // entry stays 4 + 8 bytes.  Each (part, slab) segment starts at a multiple
 // of kAlign entries (the gaps hold zeroed filler that is never summed).
 // Inside a segment the entries are ordered by column (a stable segmented
@@ -27,9 +27,15 @@
 // order): the lanes of a wave then gather nearby columns of x, so a wave's
 // gathers touch fewer cache lines and consecutive waves of a workgroup reuse
 // L1 lines.  On the 2M-row power-law matrix that took the main kernel from
-// 235 to 204 us (with 512 parts and 4 entries per thread, the best of a
-// sweep over 384..1024 parts, 1..8 entries per thread, 256/512 threads and
-// 1/2 MB slabs).
+// 235 to 204 us (512 parts, 4 consecutive entries per thread).
+//
+// Default shape: one part per CU, whose row partials may take up to 156 KB
+// of the CU's 160 KB of LDS, as one workgroup of 512 threads.  Twice the
+// entries per (part, slab) segment as with two parts per CU means twice
+// the entries per line of x under a wave's load: 171 us on the power-law
+// matrix, against 195 us for 512 parts of 256 threads (sweeps over
+// 256..1024 parts, 1..16 entries per thread, 256..1024 threads, 0.5..2 MB
+// slabs).
 //
 // Three forms of the main kernel accumulate the row partials in LDS:
 //   k_slab_spmv_il (default, 8 entries per thread): the workgroup takes
@@ -37,7 +43,7 @@
 //     ..., so one load instruction of a wave covers 64 consecutive entries
 //     of the column-ordered segment and lanes whose columns share a line of
 //     x are served by one L2 request; each product is added to its row with
-//     an LDS atomic (195 us on the power-law matrix);
+//     an LDS atomic (171 us on the power-law matrix);
 //   k_slab_spmv: a thread takes U consecutive entries (vector loads), sums
 //     the runs of one row among them and adds each run with an LDS atomic —
 //     with column-ordered segments nearly every entry is its own run (205 us
@@ -67,19 +73,24 @@ namespace taco_hip {
 
 namespace {
 
-constexpr int kColBits = 18;
+constexpr int kColBits = 17;
 constexpr uint32_t kColMask = (1u << kColBits) - 1;
+// Workgroup size of the main kernel: 512 threads when each CU holds one
+// part, 256 when it holds several.
+constexpr int kPlanThreadsOnePerCu = 512;
 constexpr int kPlanThreads = 256;
 constexpr int kDefaultUnroll = 4;     // k_slab_spmv, k_slab_spmv_wave
 constexpr int kDefaultUnrollIl = 8;   // k_slab_spmv_il
-constexpr int kDefaultParts = 512;
+// Default parts: one per CU, more (a multiple of the CU count) when the
+// rows would otherwise crowd the LDS (parts of at most half its capacity
+// on average, so that entry-balanced parts stay within it).
 // 2^17 columns = 1 MB of f64 x per slab (a quarter of an XCD's L2).
 constexpr int kDefaultSlabWidth = 1 << 17;
 // Segment start alignment (entries); the largest per-thread chunk.
 constexpr int kAlign = 8;
 // Row partials live in dynamic LDS sized by the plan's largest part; this
-// caps one part at 76 KB of f64 partials.
-constexpr int kMaxLocalRows = 9728;
+// caps one part at 156 KB of f64 partials (a CU has 160 KB of LDS).
+constexpr int kMaxLocalRows = 19968;
 
 // Largest r in [0, m] with w*r + pos[r] <= d.
 __device__ int64_t weighted_row(const int32_t* pos, int64_t m, int64_t w,
@@ -224,6 +235,15 @@ __global__ __launch_bounds__(256) void k_slab_scatter(
 // different slabs, which spreads their concurrent misses over the memory
 // side instead of all XCDs fetching the same slab at once.
 constexpr int kXcds = 8;
+
+int device_cu_count() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      n <= 0)
+    return 256;
+  return n;
+}
 __device__ __forceinline__ int32_t first_slab(int32_t part, int32_t slabs) {
   return (int32_t)((int64_t)(part % kXcds) * slabs / kXcds);
 }
@@ -623,8 +643,13 @@ extern "C" int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
   if (int rc = view_csr(A, "A", &a)) return rc;
   TH_REQUIRE(a.vt == VType::F64, "spmv plan: f64 values only");
   if (slab_width <= 0) slab_width = kDefaultSlabWidth;
-  TH_REQUIRE(slab_width <= (1 << kColBits), "spmv plan: slab width above 2^18");
-  if (parts <= 0) parts = kDefaultParts;
+  TH_REQUIRE(slab_width <= (1 << kColBits), "spmv plan: slab width above 2^17");
+  const int cus = device_cu_count();
+  if (parts <= 0) {
+    const int64_t per_cu_rows = (int64_t)cus * (kMaxLocalRows / 2);
+    parts = (int32_t)((int64_t)cus * (((int64_t)a.m + per_cu_rows - 1) / per_cu_rows));
+    if (parts < cus) parts = cus;
+  }
   const int32_t slabs = a.n == 0 ? 1 : (int32_t)(((int64_t)a.n + slab_width - 1) / slab_width);
   TH_REQUIRE(slabs <= 1024, "spmv plan: more than 1024 slabs");
   // Row weight w: first w = 0, parts of equal entry counts (so every
@@ -650,6 +675,7 @@ extern "C" int taco_hip_spmv_plan_create(taco_tensor_t* A, int32_t parts,
   p->parts = parts;
   p->slabs = slabs;
   p->slab_width = slab_width;
+  p->threads = parts <= cus ? kPlanThreadsOnePerCu : kPlanThreads;
   hipStream_t st = current_stream();
   auto alloc = [&](void** q, size_t bytes) {
     return hipMalloc(q, bytes < 16 ? 16 : bytes);
@@ -770,8 +796,10 @@ extern "C" int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* p,
   const size_t lds = sizeof(double) * (size_t)(p->max_local_rows > 0 ? p->max_local_rows : 1);
   const int u = unroll > 0 ? unroll : p->form == 2 ? kDefaultUnrollIl : kDefaultUnroll;
   const int th = p->threads;
-  if (u != 1 && u != 2 && u != 4 && u != 8 && !(u == 16 && p->form == 2 && th == 256))
+  if (u != 1 && u != 2 && u != 4 && u != 8 && !(u == 16 && p->form == 2))
     return fail("spmv plan: unroll %d unsupported", unroll);
+  TH_REQUIRE(th != 1024 || (p->form == 2 && u >= 4),
+             "spmv plan: 1024 threads for the interleaved form with 4, 8 or 16 entries per thread");
   const bool timed = p->timing_count < p->timing_cap;
   if (timed) TH_CHECK(hipEventRecord(p->timing_ev[2 * (size_t)p->timing_count], st));
 #define TH_SLAB(U, T)                                                           \
@@ -794,7 +822,8 @@ extern "C" int taco_hip_spmv_plan_compute(taco_hip_spmv_plan* p,
     TH_SLAB_W(1, 512) TH_SLAB_W(2, 512) TH_SLAB_W(4, 512) TH_SLAB_W(8, 512)
   } else if (p->form == 2) {
     TH_SLAB_I(1, 256) TH_SLAB_I(2, 256) TH_SLAB_I(4, 256) TH_SLAB_I(8, 256) TH_SLAB_I(16, 256)
-    TH_SLAB_I(1, 512) TH_SLAB_I(2, 512) TH_SLAB_I(4, 512) TH_SLAB_I(8, 512)
+    TH_SLAB_I(1, 512) TH_SLAB_I(2, 512) TH_SLAB_I(4, 512) TH_SLAB_I(8, 512) TH_SLAB_I(16, 512)
+    TH_SLAB_I(4, 1024) TH_SLAB_I(8, 1024) TH_SLAB_I(16, 1024)
   } else {
     TH_SLAB(1, 256) TH_SLAB(2, 256) TH_SLAB(4, 256) TH_SLAB(8, 256)
     TH_SLAB(1, 512) TH_SLAB(2, 512) TH_SLAB(4, 512) TH_SLAB(8, 512)
@@ -815,8 +844,9 @@ extern "C" void taco_hip_spmv_plan_destroy(taco_hip_spmv_plan* p) { plan_free(p)
 extern "C" int taco_hip_spmv_plan_configure(taco_hip_spmv_plan* p, int32_t threads,
                                            int32_t form) {
   TH_REQUIRE(p, "spmv plan: null plan");
-  if (threads <= 0) threads = kPlanThreads;
-  TH_REQUIRE(threads == 256 || threads == 512, "spmv plan: threads must be 256 or 512");
+  if (threads <= 0) threads = p->parts <= device_cu_count() ? kPlanThreadsOnePerCu : kPlanThreads;
+  TH_REQUIRE(threads == 256 || threads == 512 || threads == 1024,
+             "spmv plan: threads must be 256, 512 or 1024");
   p->threads = threads;
   TH_REQUIRE(form >= 0 && form <= 2, "spmv plan: kernel form %d unknown", form);
   p->form = form;
@@ -850,6 +880,15 @@ extern "C" int taco_hip_spmv_plan_kernel_time(taco_hip_spmv_plan* p, double* tot
   }
   *total_ms = sum;
   *launches = p->timing_count;
+  return 0;
+}
+
+extern "C" int taco_hip_spmv_plan_shape(const taco_hip_spmv_plan* p, int32_t* threads,
+                                        int32_t* form, int32_t* unroll) {
+  TH_REQUIRE(p && threads && form && unroll, "spmv plan: null argument");
+  *threads = p->threads;
+  *form = p->form;
+  *unroll = p->form == 2 ? kDefaultUnrollIl : kDefaultUnroll;
   return 0;
 }
 

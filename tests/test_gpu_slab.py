@@ -75,16 +75,38 @@ def test_slab_plan_rejects_f32():
 
 
 def test_slab_plan_library_defaults():
-    """The plan as the library builds it by default: 512 parts, segments
-    ordered by column, interleaved entries, 8 per thread."""
+    """The plan as the library builds it by default: one part per CU,
+    segments ordered by column, interleaved entries, 8 per thread, 512
+    threads."""
     A = W.csr_powerlaw(200000, 180000, seed=48, kmax=50000)
     x = W.uniform(A.n, 7, W.STREAM_X)
     a = G.csr_desc(A)
     plan = _lib.SpmvPlan(a)
-    assert plan.info()["parts"] == 512
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    assert plan.info()["parts"] == cus
+    assert plan.shape() == {"threads": 512, "form": 2, "unroll": 8}
     y = torch.full((A.m,), float("nan"), dtype=torch.float64, device="cuda")
     plan.compute(D.dense(y), D.dense(G.dev(x)))
     torch.cuda.synchronize()
     ye, cond = oracle.exact_spmv(A, x)
     rep = oracle.error_report(y.cpu().numpy(), ye, cond, 1e-12)
     assert rep["ok"], rep
+
+
+def test_slab_plan_default_parts_grow_with_rows():
+    """Rows beyond half the LDS capacity of one part per CU: the default
+    takes the next multiple of the CU count (here two parts per CU, at 256
+    threads), still within the FP64 oracle's tolerance."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    m = cus * 9984 + 1000
+    A = W.csr_from_counts(m, 50000, W.uniform_counts(m, 0, 4, 49), 49)
+    x = W.uniform(A.n, 8, W.STREAM_X)
+    print(f"slab plan default parts m={m} nnz={A.nnz}", flush=True)
+    plan = _lib.SpmvPlan(G.csr_desc(A))
+    assert plan.info()["parts"] == 2 * cus
+    assert plan.shape()["threads"] == 256
+    y = torch.full((A.m,), float("nan"), dtype=torch.float64, device="cuda")
+    plan.compute(D.dense(y), D.dense(G.dev(x)))
+    torch.cuda.synchronize()
+    ye, cond = oracle.exact_spmv(A, x)
+    assert oracle.error_report(y.cpu().numpy(), ye, cond, 1e-12)["ok"]
