@@ -54,6 +54,20 @@ __global__ __launch_bounds__(256) void k_mfma_parts(
   }
 }
 
+// Task -> row group map: a task's wave reads its group with one load instead
+// of a binary search over part_pos (17 dependent loads at two million rows,
+// about as long as a typical task's own gathers).  Group g owns the tasks
+// part_pos[g] .. part_pos[g+1]-1; every group has at least one.
+__global__ __launch_bounds__(256) void k_mfma_task_groups(
+    int32_t ngroups, int64_t max_tasks, const int32_t* __restrict__ part_pos,
+    int32_t* __restrict__ task_group) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  const int64_t s = part_pos[g];
+  const int64_t e = (int64_t)part_pos[g + 1] < max_tasks ? part_pos[g + 1] : max_tasks;
+  for (int64_t t = s; t < e; ++t) task_group[t] = (int32_t)g;
+}
+
 __device__ __forceinline__ short bf16_or_zero(const uint16_t* p, bool ok) {
   return ok ? (short)p[0] : (short)0;
 }
@@ -63,6 +77,7 @@ __device__ __forceinline__ short bf16_or_zero(const uint16_t* p, bool ok) {
 // choice (5) took 1.01 ms.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_spmm_mfma_bf16(
     int32_t m, int32_t K, int32_t ngroups, const int32_t* __restrict__ part_pos,
+    const int32_t* __restrict__ task_group,
     const int32_t* __restrict__ pos, const int32_t* __restrict__ crd,
     const uint16_t* __restrict__ vals, const uint16_t* __restrict__ B,
     float* __restrict__ C, bool coarse) {
@@ -71,13 +86,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
   const int64_t t = (int64_t)blockIdx.x * 4 + wave;
   const int64_t ntasks = part_pos[ngroups];
   if (t >= ntasks) return;
-  // Group of task t: largest g with part_pos[g] <= t.
-  int32_t lo = 0, hi = ngroups;
-  while (hi - lo > 1) {
-    const int32_t mid = lo + (hi - lo) / 2;
-    if ((int64_t)part_pos[mid] <= t) lo = mid; else hi = mid;
-  }
-  const int32_t g = lo;
+  const int32_t g = task_group[t];
   const int32_t part = (int32_t)(t - part_pos[g]);
   const int32_t nparts = part_pos[g + 1] - part_pos[g];
   const int64_t r0 = (int64_t)g * 16;
@@ -171,17 +180,23 @@ int spmm_mfma_bf16(const CsrView& a, int32_t K, const void* B, float* C,
              "spmm: the MFMA path needs 8-byte aligned B and 16-byte aligned C");
   hipStream_t st = current_stream();
   const int32_t ngroups = (int32_t)(((int64_t)a.m + 15) / 16);
-  char* ws = (char*)workspace(sizeof(int32_t) * (2 * (size_t)ngroups + 2) + 512, 1);
+  // Every group has ceil(T/kTileMax) parts, at least one, so the tasks
+  // number at most ngroups + ceil(nnz/kTileMax).
+  const int64_t max_tasks = (int64_t)ngroups + (a.nnz + kTileMax - 1) / kTileMax;
+  char* ws = (char*)workspace(
+      sizeof(int32_t) * (2 * (size_t)ngroups + 2 + (size_t)max_tasks) + 512, 1);
   if (!ws) return 1;
   int32_t* status = (int32_t*)ws;
   int32_t* parts = (int32_t*)(ws + 256);
   int32_t* part_pos = parts + ngroups + 1;
+  int32_t* task_group = part_pos + ngroups + 1;
   k_mfma_parts<<<(ngroups + 255) / 256, 256, 0, st>>>(a.m, K, a.pos, parts, C, status);
   if (int rc = device_counts_to_pos(ngroups, parts, part_pos, status, st))
     return rc;
-  const int64_t max_tasks = (int64_t)ngroups + (a.nnz + kTileMax - 1) / kTileMax;
+  k_mfma_task_groups<<<(ngroups + 255) / 256, 256, 0, st>>>(ngroups, max_tasks,
+                                                          part_pos, task_group);
   const unsigned grid = (unsigned)((max_tasks + 3) / 4);
-  k_spmm_mfma_bf16<<<grid, 256, 0, st>>>(a.m, K, ngroups, part_pos, a.pos,
+  k_spmm_mfma_bf16<<<grid, 256, 0, st>>>(a.m, K, ngroups, part_pos, task_group, a.pos,
                                          a.crd, (const uint16_t*)a.vals,
                                          (const uint16_t*)B, C,
                                          coarse_grained(C));
