@@ -73,8 +73,12 @@ __device__ __forceinline__ short bf16_or_zero(const uint16_t* p, bool ok) {
 }
 
 // 7 waves per SIMD: measured 0.89 ms on the 2M-row power-law matrix (K=64)
-// against 0.92 ms at 6 (no spill) and 1.28 ms at 8; the compiler's own
-// choice (5) took 1.01 ms.
+// against 0.92 ms at 6 and 1.28 ms at 8; the compiler's own choice (5) took
+// 1.01 ms.  Entry indices are 32-bit (CSR positions are int32) and, when B
+// spans less than 4 GiB (kOff32), so are B's byte offsets: global loads then
+// take a scalar base plus one offset register, which keeps the kernel within
+// 7 waves' registers without spilling.
+template <bool kOff32>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_spmm_mfma_bf16(
     int32_t m, int32_t K, int32_t ngroups, const int32_t* __restrict__ part_pos,
     const int32_t* __restrict__ task_group,
@@ -83,21 +87,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     float* __restrict__ C, bool coarse) {
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
-  const int64_t t = (int64_t)blockIdx.x * 4 + wave;
-  const int64_t ntasks = part_pos[ngroups];
+  const int32_t t = (int32_t)blockIdx.x * 4 + wave;
+  const int32_t ntasks = part_pos[ngroups];
   if (t >= ntasks) return;
   const int32_t g = task_group[t];
-  const int32_t part = (int32_t)(t - part_pos[g]);
+  const int32_t part = t - part_pos[g];
   const int32_t nparts = part_pos[g + 1] - part_pos[g];
-  const int64_t r0 = (int64_t)g * 16;
-  const int rows = m - r0 < 16 ? (int)(m - r0) : 16;
+  const int32_t r0 = g * 16;
+  const int rows = m - r0 < 16 ? m - r0 : 16;
   const int mrow = lane & 15;
   const int kq = lane >> 4;
-  const int64_t ps = mrow < rows ? pos[r0 + mrow] : 0;
-  const int64_t pe = mrow < rows ? pos[r0 + mrow + 1] : 0;
-  const int64_t gs = pos[r0], ge = pos[r0 + rows];
-  const int64_t es = gs + (int64_t)part * kTileMax;
-  const int64_t ee = es + kTileMax < ge ? es + kTileMax : ge;
+  const int32_t ps = mrow < rows ? pos[r0 + mrow] : 0;
+  const int32_t pe = mrow < rows ? pos[r0 + mrow + 1] : 0;
+  const int32_t gs = pos[r0], ge = pos[r0 + rows];
+  const int32_t es = gs + part * kTileMax;
+  const int32_t ee = ge - es < kTileMax ? ge : es + kTileMax;
+  const char* Bb = (const char*)B;
 
   for (int32_t c0 = 0; c0 < K; c0 += 64) {
     const int32_t col = c0 + 4 * (lane & 15);
@@ -112,12 +117,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     v4i16 an;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int64_t idx = es + 4 * kq + q;
+      const int32_t idx = es + 4 * kq + q;
       const bool valid = idx < ee;
       jn[q] = valid ? stream_load(crd + idx) : 0;
       an[q] = bf16_or_zero(vals + idx, valid && idx >= ps && idx < pe);
     }
-    for (int64_t e = es; e < ee; e += 16) {
+    for (int32_t e = es; e < ee; e += 16) {
       const v4i16 a = an;
       int32_t j[4];
 #pragma unroll
@@ -125,13 +130,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
       uint2 bq[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int64_t idx = e + 4 * kq + q;
-        bq[q] = (idx < ee && col_ok) ? *(const uint2*)(B + (int64_t)j[q] * K + col)
-                                     : make_uint2(0u, 0u);
+        const int32_t idx = e + 4 * kq + q;
+        const bool ok = idx < ee && col_ok;
+        if constexpr (kOff32) {
+          const uint32_t off = ((uint32_t)j[q] * (uint32_t)K + (uint32_t)col) * 2u;
+          bq[q] = ok ? *(const uint2*)(Bb + off) : make_uint2(0u, 0u);
+        } else {
+          bq[q] = ok ? *(const uint2*)(B + (int64_t)j[q] * K + col) : make_uint2(0u, 0u);
+        }
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int64_t idx = e + 16 + 4 * kq + q;
+        const int32_t idx = e + 16 + 4 * kq + q;
         const bool valid = idx < ee;
         jn[q] = valid ? stream_load(crd + idx) : 0;
         an[q] = bf16_or_zero(vals + idx, valid && idx >= ps && idx < pe);
@@ -152,7 +162,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     for (int r = 0; r < 4; ++r) {
       const int lr = 4 * kq + r;
       if (lr >= rows) continue;
-      float* dst = C + (r0 + lr) * K + col;
+      float* dst = C + ((int64_t)r0 + lr) * K + col;
       if (nparts == 1) {
         // Non-temporal: C is written once; keeping it out of the caches
         // leaves them to the gathered B rows.
@@ -196,10 +206,10 @@ int spmm_mfma_bf16(const CsrView& a, int32_t K, const void* B, float* C,
   k_mfma_task_groups<<<(ngroups + 255) / 256, 256, 0, st>>>(ngroups, max_tasks,
                                                           part_pos, task_group);
   const unsigned grid = (unsigned)((max_tasks + 3) / 4);
-  k_spmm_mfma_bf16<<<grid, 256, 0, st>>>(a.m, K, ngroups, part_pos, task_group, a.pos,
-                                         a.crd, (const uint16_t*)a.vals,
-                                         (const uint16_t*)B, C,
-                                         coarse_grained(C));
+  const bool off32 = (int64_t)a.n * K * 2 <= (int64_t)UINT32_MAX;
+  (off32 ? k_spmm_mfma_bf16<true> : k_spmm_mfma_bf16<false>)<<<grid, 256, 0, st>>>(
+      a.m, K, ngroups, part_pos, task_group, a.pos, a.crd, (const uint16_t*)a.vals,
+      (const uint16_t*)B, C, coarse_grained(C));
   TH_LAUNCH_CHECK("k_spmm_mfma_bf16");
   return 0;
 }

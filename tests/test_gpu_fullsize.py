@@ -115,3 +115,33 @@ def test_spgemm_500k_pattern_bit_exact():
     ex, cond = oracle.exact_spgemm_values(A, B, opos, ocrd)
     assert oracle.error_report(vals, ex, cond, 1e-12)["ok"]
     assert oracle.error_report(ovals, ex, cond, 1e-12)["ok"]
+
+
+def test_spmm_mfma_b_beyond_4gib():
+    """bf16 SpMM on the matrix cores with B spanning more than 4 GiB (2^26+64
+    rows of 64 columns, 8.6 GB): row offsets of B no longer fit 32 bits and
+    the kernel takes its 64-bit addressing.  Half of A's columns lie past
+    row 2^25, where a 32-bit byte offset would wrap.  The oracle runs on the
+    rows of B that A touches."""
+    K = 64
+    n = (1 << 26) + 64
+    A = W.csr_fixed(3000, n, 8, 71)
+    print("spmm bf16, B of", n, "rows", flush=True)
+    Bd = torch.empty((n, K), dtype=torch.bfloat16, device="cuda").uniform_(-1.0, 1.0)
+    a = G.csr_desc(A, "bf16")
+    C = torch.full((A.m, K), float("nan"), dtype=torch.float32, device="cuda")
+    bdesc, cdesc = D.dense(Bd.view(torch.int16), "bf16"), D.dense(C, "f32")
+    _lib.check(_lib.lib().taco_hip_spmm_compute(cdesc.ptr, a.ptr, bdesc.ptr,
+                                                G.sched(_lib.MFMA)))
+    torch.cuda.synchronize()
+    got = C.cpu().numpy().astype(np.float64)
+    uniq = np.unique(A.crd)
+    assert uniq[-1] >= (1 << 25)
+    rows = Bd[torch.from_numpy(uniq.astype(np.int64)).cuda()].double().cpu().numpy()
+    del bdesc, Bd
+    torch.cuda.empty_cache()
+    Ar = W.Csr(A.m, len(uniq), A.pos, np.searchsorted(uniq, A.crd).astype(np.int32),
+               G.bf16_values(A.vals))
+    Ce, cond = oracle.exact_spmm(Ar, rows)
+    rep = oracle.error_report(got, Ce, cond, 1e-5)
+    assert rep["ok"], rep
