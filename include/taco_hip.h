@@ -48,7 +48,12 @@ const char* taco_hip_get_default_compiler_flags(void);
 void* taco_hip_unified_alloc(size_t size);
 /* cuda_unified_free (cuda.cpp:66-72).  Frees memory from
  * taco_hip_unified_alloc or any allocation this library hands out (outputs
- * of assemble functions). */
+ * of assemble functions).  Like hipFree it synchronizes the device.  Device
+ * blocks of 1 MiB and more (sizes rounded up to 2 MiB) are kept, up to 32
+ * blocks and 32 GiB per process, and handed out again by the next
+ * taco_hip_unified_alloc of the same rounded size; an allocation that finds
+ * the device out of memory returns the kept blocks first.  Managed blocks
+ * are never kept. */
 void taco_hip_unified_free(void* ptr);
 /* Reallocation for generated host code (taco's realloc emulation,
  * codegen_cuda.cpp:1086-1136): a new taco_hip_unified_alloc block holding the

@@ -266,16 +266,95 @@ static bool has_device() {
   return n > 0;
 }
 
+// Cache of large device blocks freed through taco_hip_unified_free.  taco
+// allocates every output array anew on each assemble (SpGEMM's C is 384 MB
+// at 32M entries), and hipMalloc/hipFree of blocks that size cost 60-300 us
+// each.  Blocks of 1 MiB and more (sizes rounded up to 2 MiB) are kept, per
+// device, up to kCacheBlocks blocks and kCacheBytes bytes, and handed out
+// again for an allocation of the same rounded size.  A free synchronizes
+// the device first, as hipFree does, so no kernel still reads a block that
+// is handed out again.  Managed allocations are never cached.
+namespace {
+constexpr size_t kCacheMinBytes = size_t(1) << 20;
+constexpr size_t kCacheGranule = size_t(2) << 20;
+constexpr size_t kCacheBlocks = 32;
+constexpr size_t kCacheBytes = size_t(32) << 30;
+
+struct DeviceBlock {
+  void* p;
+  size_t bytes;
+  int device;
+};
+
+std::mutex g_cache_mu;
+std::vector<DeviceBlock> g_live;    // cacheable blocks handed out
+std::vector<DeviceBlock> g_cached;  // freed blocks kept for reuse
+size_t g_cached_bytes = 0;
+
+size_t cache_round(size_t size) {
+  return (size + kCacheGranule - 1) / kCacheGranule * kCacheGranule;
+}
+
+void cache_release_all() {  // caller holds g_cache_mu
+  for (const DeviceBlock& b : g_cached) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(b.device);
+    (void)hipFree(b.p);
+    (void)hipSetDevice(cur);
+  }
+  g_cached.clear();
+  g_cached_bytes = 0;
+}
+}  // namespace
+
 void* taco_hip_unified_alloc(size_t size) {
   if (size == 0) return nullptr;
   if (!has_device()) return std::malloc(size);
   void* p = nullptr;
-  hipError_t e = g_unified_memory ? hipMallocManaged(&p, size)
-                                  : hipMalloc(&p, size);
+  if (g_unified_memory) {
+    hipError_t e = hipMallocManaged(&p, size);
+    if (e != hipSuccess) {
+      hip_fail(e, "taco_hip_unified_alloc", __FILE__, __LINE__);
+      return nullptr;
+    }
+    return p;
+  }
+  if (size < kCacheMinBytes) {
+    hipError_t e = hipMalloc(&p, size);
+    if (e != hipSuccess) {
+      hip_fail(e, "taco_hip_unified_alloc", __FILE__, __LINE__);
+      return nullptr;
+    }
+    return p;
+  }
+  const size_t bytes = cache_round(size);
+  int dev = 0;
+  if (hipError_t e = hipGetDevice(&dev); e != hipSuccess) {
+    hip_fail(e, "taco_hip_unified_alloc", __FILE__, __LINE__);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  for (size_t k = g_cached.size(); k-- > 0;) {  // most recently freed first
+    if (g_cached[k].bytes == bytes && g_cached[k].device == dev) {
+      const DeviceBlock b = g_cached[k];
+      g_cached.erase(g_cached.begin() + (std::ptrdiff_t)k);
+      g_cached_bytes -= b.bytes;
+      g_live.push_back(b);
+      return b.p;
+    }
+  }
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess && !g_cached.empty()) {
+    (void)hipGetLastError();  // out of memory: return the cache, try again
+    cache_release_all();
+    e = hipMalloc(&p, bytes);
+  }
   if (e != hipSuccess) {
     hip_fail(e, "taco_hip_unified_alloc", __FILE__, __LINE__);
     return nullptr;
   }
+  g_live.push_back(DeviceBlock{p, bytes, dev});
   return p;
 }
 
@@ -284,6 +363,29 @@ void taco_hip_unified_free(void* ptr) {
   if (!has_device()) {
     std::free(ptr);
     return;
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    for (size_t k = 0; k < g_live.size(); ++k) {
+      if (g_live[k].p != ptr) continue;
+      const DeviceBlock b = g_live[k];
+      g_live.erase(g_live.begin() + (std::ptrdiff_t)k);
+      int cur = 0;
+      hipError_t e = hipGetDevice(&cur);
+      if (e == hipSuccess) e = hipSetDevice(b.device);
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      (void)hipSetDevice(cur);
+      if (e != hipSuccess) {
+        hip_fail(e, "taco_hip_unified_free", __FILE__, __LINE__);
+        return;
+      }
+      if (g_cached.size() < kCacheBlocks && g_cached_bytes + b.bytes <= kCacheBytes) {
+        g_cached.push_back(b);
+        g_cached_bytes += b.bytes;
+        return;
+      }
+      break;  // the cache is full: free the block itself
+    }
   }
   hipError_t e = hipFree(ptr);
   if (e != hipSuccess) hip_fail(e, "taco_hip_unified_free", __FILE__, __LINE__);
